@@ -1,0 +1,144 @@
+/*
+ * qx.h -- C ABI of the MI355X (gfx950) kick-drift-kick Schroedinger-Poisson
+ * library (libqx.so).  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * One qx_ctx stands for one reference `Field` (src/field.h:13-97): it owns
+ * the device-resident state psi (S complex), V (S real) and the potential's
+ * half spectrum, the FFT twiddles, one HIP stream and scratch.  Every call
+ * returns after its work is complete on the stream, which keeps the
+ * reference's synchronous semantics.  Calls on one context must be
+ * serialised by the caller; distinct contexts may be used from different
+ * threads.  Errors: a status code plus qx_last_error() (thread-local text).
+ *
+ * Host arrays are caller-owned and read/written only during the call.  At
+ * this boundary psi is always complex128 interleaved (re, im), V float64,
+ * half spectra complex128 in the reference layout [i0][i1][k_last] with
+ * k_last in [0, N/2] (src/fft/fft.h:63-64); in QX_FP32 contexts the device
+ * state is complex64/float32 and converted on the device.
+ *
+ * Lattices: dim 2 or 3, N a power of two in [2, 4096] (2D) or [2, 1024]
+ * (3D).  Other N are rejected with QX_EINVAL.
+ */
+#ifndef QX_H
+#define QX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qx_ctx qx_ctx;
+
+typedef enum {
+    QX_OK = 0,
+    QX_EINVAL = 1,   /* bad argument (reference: std::invalid_argument)      */
+    QX_ENOMEM = 2,   /* device allocation failed (reference: std::bad_alloc) */
+    QX_EDEVICE = 3,  /* HIP runtime error (reference: std::runtime_error)     */
+    QX_ESTATE = 4    /* call not valid in the current state                   */
+} qx_status;
+
+typedef enum { QX_FP32 = 0, QX_FP64 = 1 } qx_precision;
+
+/* Physics constants of the step.
+ *   pref     = norm_ * a (src/field.cxx:298), the Poisson source prefactor;
+ *   rho_bar  = rho_mean_ (src/field.cxx:78,86), subtracted from |psi|^2;
+ *   k0_mode  = 1: the k = 0 mode of Vhat is multiplied by -1/S as in the
+ *              reference (src/field.cxx:253-254,282-283); 0: it is zeroed;
+ *   rho_bar_mode = 1: subtract the actual mean <|psi|^2> (k = 0 zeroed). */
+typedef struct {
+    double pref;
+    double rho_bar;
+    int k0_mode;
+    int rho_bar_mode;
+} qx_physics;
+
+/* ---- lifetime: Field::Field/init/~Field (src/field.cxx:13-66) ----------- */
+qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_ctx **out);
+qx_status qx_destroy(qx_ctx *ctx);
+/* setCosmo STATIC branch (src/field.cxx:73-80) sets pref = 4 pi norm, rho_bar = 1 */
+qx_status qx_set_physics(qx_ctx *ctx, const qx_physics *ph);
+qx_status qx_get_physics(const qx_ctx *ctx, qx_physics *ph);
+
+/* ---- state transfer: Field::psi()/V() accessors (src/field.h:36-45) ----- */
+qx_status qx_upload_psi(qx_ctx *ctx, const double *psi_c128);
+qx_status qx_download_psi(qx_ctx *ctx, double *psi_c128);
+qx_status qx_upload_V(qx_ctx *ctx, const double *V);
+qx_status qx_download_V(qx_ctx *ctx, double *V);
+/* device pointers of psi / V (element type set by the precision) */
+qx_status qx_device_state(qx_ctx *ctx, void **psi, void **V);
+
+/* ---- split-step operators ------------------------------------------------ */
+/* Field::kick (src/field.cxx:93-116): psi <- psi exp(-i dt V) */
+qx_status qx_kick(qx_ctx *ctx, double dt);
+/* Field::drift (src/field.cxx:119-198): psi <- IFFT(e^{-i dt k^2/2}/S FFT psi) */
+qx_status qx_drift(qx_ctx *ctx, double dt);
+/* Field::updatePotential (src/field.cxx:294-346); vmax nullable */
+qx_status qx_update_potential(qx_ctx *ctx, double *vmax);
+/* Field::Poisson (src/field.cxx:230-291) applied to a caller half spectrum
+ * (reference layout, complex128, modified in place) */
+qx_status qx_poisson_half(qx_ctx *ctx, double *vhat_c128);
+/* Field::computeEnergy (src/field.cxx:200-227): V <- |psi|^2, rho_max/mean */
+qx_status qx_compute_energy(qx_ctx *ctx, double *rhomax_over_mean);
+/* density statistics that leave V untouched; outputs nullable.  mass =
+ * (L/N)^dim sum |psi|^2. */
+qx_status qx_density_stats(qx_ctx *ctx, double *rhomax_over_mean, double *mass);
+/* mass, kinetic 1/2 dV/S sum k^2|psihat|^2 and potential 1/2 dV sum V|psi|^2
+ * (no reference counterpart; outputs nullable) */
+qx_status qx_energy(qx_ctx *ctx, double *mass, double *kinetic, double *potential);
+/* Vmax of the last potential update (Field::Vmax, src/field.h:56) */
+double qx_vmax(const qx_ctx *ctx);
+
+/* Spectrum::compute (src/spectrum/spectrum.cxx:32-164).  source 0: the real
+ * field V as it stands (the reference's choice); 1: |psi|^2 computed on the
+ * fly, leaving V untouched.  nbins/spb: bin j gathers the integer shells
+ * [j*spb, (j+1)*spb); the reference is nbins = N/2+1, spb = 1.  Outputs
+ * (arrays of nbins): shell-averaged P, mode counts, k of the bin. */
+qx_status qx_spectrum(qx_ctx *ctx, int source, int nbins, int spb, double *Pk, int64_t *count,
+                      double *kmodes);
+
+/* ---- step compositions ---------------------------------------------------
+ * One merged-kick leapfrog segment of the reference time loop
+ * (src/main.cxx:65-85): requires V current (after qx_update_potential).
+ * Opens with K(ds/2) if first_half else K(ds) (continuing a segment left
+ * open), then nsteps x [D(ds), P] with K(ds) between steps, and closes with
+ * K(ds/2) if last_half.  Leaves V current.  Runs the fused pass schedule. */
+qx_status qx_leapfrog(qx_ctx *ctx, double ds, int nsteps, int first_half, int last_half);
+/* Field::propagate (src/field.cxx:388-402): K(ds/2) D(ds) P K(ds/2) */
+qx_status qx_propagate(qx_ctx *ctx, double ds);
+
+/* ---- FFTBackend (src/fft/fft.h:9-26), host arrays ---------------------- */
+/* forward_c2c / backward_c2c (src/fft/fft.h:122-130): in place, S complex,
+ * sign -1 forward, +1 backward, unnormalised */
+qx_status qx_fft_c2c(qx_ctx *ctx, double *data_c128, int sign);
+/* forward_r2c (src/fft/fft.h:132-135): S real -> H complex half spectrum */
+qx_status qx_fft_r2c(qx_ctx *ctx, const double *in, double *out_c128);
+/* backward_c2r (src/fft/fft.h:137-140): H complex -> S real; `in` is not
+ * modified (FFTW may destroy it, so callers cannot rely on either) */
+qx_status qx_fft_c2r(qx_ctx *ctx, const double *in_c128, double *out);
+/* sites() / ksites() / global_N() (src/fft/fft.h:19-25) */
+int64_t qx_sites(const qx_ctx *ctx);
+int64_t qx_ksites(const qx_ctx *ctx);
+int qx_global_N(const qx_ctx *ctx);
+
+/* ---- initial conditions on the device (no reference counterpart except
+ * qx_ic_solitons, SolitonsIC::apply src/ic/solitons.cxx:20-110) ----------- */
+/* solitons: rows (amp, sigma, x, y, z) in box fractions, background bkg */
+qx_status qx_ic_solitons(qx_ctx *ctx, const double *sol, int nsol, double bkg);
+/* Gaussian random field: delta with P(k) ~ k^-2 for 0 < |n| <= N/4,
+ * psi = sqrt(max(0, 1 + amp delta/sigma)) e^{i theta}, theta = U[0,2pi)
+ * noise low-passed at the same cutoff; counter-based RNG keyed by seed */
+qx_status qx_ic_grf(qx_ctx *ctx, uint64_t seed, double amp);
+/* blobs: rows (rho_c, r_c, x, y, z, phi), background density bkg */
+qx_status qx_ic_blobs(qx_ctx *ctx, const double *table, int nblobs, double bkg);
+
+/* ---- plumbing (no reference counterpart) -------------------------------- */
+void *qx_stream(qx_ctx *ctx);          /* the context's hipStream_t */
+qx_status qx_synchronize(qx_ctx *ctx);
+const char *qx_last_error(void);
+const char *qx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,311 @@
+"""qaxions_amd -- MI355X-native kick-drift-kick Schroedinger-Poisson step.
+
+Python access to ``libqx.so`` (HIP kernels for gfx950 behind the C ABI in
+``include/qx.h``).  There is no CPU fallback: if the library is missing or no
+GPU is visible, every entry point raises.
+
+    >>> from qaxions_amd import Lattice
+    >>> lat = Lattice(dim=3, N=64, L=20.0, precision="fp32")
+    >>> lat.ic_grf(seed=0)
+    >>> lat.update_potential()
+    >>> lat.leapfrog(ds=5e-4, nsteps=10)
+    >>> psi = lat.psi          # complex128 numpy array, shape (N,)*dim
+
+``Lattice`` is the Python face of one reference ``Field`` (src/field.h:13-97);
+the pybind11 module ``qaxions_amd.fieldtest`` keeps the reference's own
+test entry points (src/examples/fieldtest.cxx:99-105).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+__all__ = ["Lattice", "QxError", "lib", "LIB_PATH", "physics_static"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqx.so")
+
+QX_OK, QX_EINVAL, QX_ENOMEM, QX_EDEVICE, QX_ESTATE = range(5)
+
+
+class QxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"qx status {status}: {msg}")
+        self.status = status
+
+
+class Physics(C.Structure):
+    _fields_ = [("pref", C.c_double), ("rho_bar", C.c_double),
+                ("k0_mode", C.c_int), ("rho_bar_mode", C.c_int)]
+
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_lib = None
+
+# every symbol include/qx.h declares, with its ctypes signature
+SIGNATURES = {
+    "qx_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "qx_destroy": (C.c_int, [C.c_void_p]),
+    "qx_set_physics": (C.c_int, [C.c_void_p, C.POINTER(Physics)]),
+    "qx_get_physics": (C.c_int, [C.c_void_p, C.POINTER(Physics)]),
+    "qx_upload_psi": (C.c_int, [C.c_void_p, _dp]),
+    "qx_download_psi": (C.c_int, [C.c_void_p, _dp]),
+    "qx_upload_V": (C.c_int, [C.c_void_p, _dp]),
+    "qx_download_V": (C.c_int, [C.c_void_p, _dp]),
+    "qx_device_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "qx_kick": (C.c_int, [C.c_void_p, C.c_double]),
+    "qx_drift": (C.c_int, [C.c_void_p, C.c_double]),
+    "qx_update_potential": (C.c_int, [C.c_void_p, _dp]),
+    "qx_poisson_half": (C.c_int, [C.c_void_p, _dp]),
+    "qx_compute_energy": (C.c_int, [C.c_void_p, _dp]),
+    "qx_density_stats": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "qx_energy": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "qx_vmax": (C.c_double, [C.c_void_p]),
+    "qx_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _i64p, _dp]),
+    "qx_leapfrog": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
+    "qx_propagate": (C.c_int, [C.c_void_p, C.c_double]),
+    "qx_fft_c2c": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "qx_fft_r2c": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "qx_fft_c2r": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "qx_sites": (C.c_int64, [C.c_void_p]),
+    "qx_ksites": (C.c_int64, [C.c_void_p]),
+    "qx_global_N": (C.c_int, [C.c_void_p]),
+    "qx_ic_solitons": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double]),
+    "qx_ic_grf": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double]),
+    "qx_ic_blobs": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double]),
+    "qx_stream": (C.c_void_p, [C.c_void_p]),
+    "qx_synchronize": (C.c_int, [C.c_void_p]),
+    "qx_last_error": (C.c_char_p, []),
+    "qx_version": (C.c_char_p, []),
+}
+
+
+def lib():
+    """Load libqx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m qaxions_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != QX_OK:
+        raise QxError(status, lib().qx_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def physics_static(norm=1.0, k0_mode=1, rho_bar_mode=0):
+    """STATIC cosmology constants (src/field.cxx:73-80): pref = 4 pi norm, rho_bar = 1."""
+    return dict(pref=4.0 * math.pi * norm, rho_bar=1.0, k0_mode=k0_mode,
+                rho_bar_mode=rho_bar_mode)
+
+
+class Lattice:
+    """One device-resident lattice (a reference ``Field``) behind the C ABI."""
+
+    def __init__(self, dim, N, L, precision="fp32", device=0, pref=1.0, rho_bar=1.0,
+                 k0_mode=1, rho_bar_mode=0):
+        self.dim, self.N, self.L = int(dim), int(N), float(L)
+        self.precision = precision
+        prec = {"fp32": 0, "fp64": 1}[precision]
+        h = C.c_void_p()
+        _check(lib().qx_create(self.dim, self.N, self.L, prec, int(device), C.byref(h)))
+        self._h = h
+        self.set_physics(pref=pref, rho_bar=rho_bar, k0_mode=k0_mode, rho_bar_mode=rho_bar_mode)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def shape(self):
+        return (self.N,) * self.dim
+
+    @property
+    def half_shape(self):
+        return (self.N,) * (self.dim - 1) + (self.N // 2 + 1,)
+
+    @property
+    def sites(self):
+        return int(lib().qx_sites(self._h))
+
+    @property
+    def ksites(self):
+        return int(lib().qx_ksites(self._h))
+
+    # -- physics --------------------------------------------------------------
+    def set_physics(self, pref, rho_bar=1.0, k0_mode=1, rho_bar_mode=0):
+        ph = Physics(float(pref), float(rho_bar), int(k0_mode), int(rho_bar_mode))
+        _check(lib().qx_set_physics(self._h, C.byref(ph)))
+
+    def get_physics(self):
+        ph = Physics()
+        _check(lib().qx_get_physics(self._h, C.byref(ph)))
+        return dict(pref=ph.pref, rho_bar=ph.rho_bar, k0_mode=ph.k0_mode,
+                    rho_bar_mode=ph.rho_bar_mode)
+
+    # -- state ------------------------------------------------------------------
+    @property
+    def psi(self):
+        out = np.empty(self.shape, np.complex128)
+        _check(lib().qx_download_psi(self._h, _p(out.view(np.float64))))
+        return out
+
+    @psi.setter
+    def psi(self, value):
+        a = np.ascontiguousarray(value, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"psi must have shape {self.shape}")
+        _check(lib().qx_upload_psi(self._h, _p(a.view(np.float64))))
+
+    @property
+    def V(self):
+        out = np.empty(self.shape, np.float64)
+        _check(lib().qx_download_V(self._h, _p(out)))
+        return out
+
+    @V.setter
+    def V(self, value):
+        a = np.ascontiguousarray(value, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"V must have shape {self.shape}")
+        _check(lib().qx_upload_V(self._h, _p(a)))
+
+    # -- operators ---------------------------------------------------------------
+    def kick(self, dt):
+        _check(lib().qx_kick(self._h, float(dt)))
+
+    def drift(self, dt):
+        _check(lib().qx_drift(self._h, float(dt)))
+
+    def update_potential(self):
+        v = C.c_double()
+        _check(lib().qx_update_potential(self._h, C.byref(v)))
+        return v.value
+
+    def poisson_half(self, vhat):
+        a = np.ascontiguousarray(vhat, dtype=np.complex128).copy()
+        if a.shape != self.half_shape:
+            raise ValueError(f"half spectrum must have shape {self.half_shape}")
+        _check(lib().qx_poisson_half(self._h, _p(a.view(np.float64))))
+        return a
+
+    def compute_energy(self):
+        r = C.c_double()
+        _check(lib().qx_compute_energy(self._h, C.byref(r)))
+        return r.value
+
+    def density_stats(self):
+        r, m = C.c_double(), C.c_double()
+        _check(lib().qx_density_stats(self._h, C.byref(r), C.byref(m)))
+        return r.value, m.value
+
+    def energy(self):
+        m, k, p = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().qx_energy(self._h, C.byref(m), C.byref(k), C.byref(p)))
+        return m.value, k.value, p.value
+
+    @property
+    def vmax(self):
+        return float(lib().qx_vmax(self._h))
+
+    def spectrum(self, source="V", nbins=None, spb=1):
+        src = {"V": 0, "rho": 1}[source]
+        if nbins is None:
+            nbins = self.N // 2 + 1
+        P = np.zeros(nbins)
+        cnt = np.zeros(nbins, np.int64)
+        k = np.zeros(nbins)
+        _check(lib().qx_spectrum(self._h, src, int(nbins), int(spb), _p(P),
+                                 cnt.ctypes.data_as(_i64p), _p(k)))
+        return P, cnt, k
+
+    def leapfrog(self, ds, nsteps, first_half=True, last_half=True):
+        _check(lib().qx_leapfrog(self._h, float(ds), int(nsteps), int(bool(first_half)),
+                                 int(bool(last_half))))
+
+    def propagate(self, ds):
+        _check(lib().qx_propagate(self._h, float(ds)))
+
+    # -- FFTBackend ------------------------------------------------------------------
+    def fft_c2c(self, x, sign):
+        a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+        _check(lib().qx_fft_c2c(self._h, _p(a.view(np.float64)), int(sign)))
+        return a
+
+    def fft_r2c(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.half_shape, np.complex128)
+        _check(lib().qx_fft_r2c(self._h, _p(a), _p(out.view(np.float64))))
+        return out
+
+    def fft_c2r(self, xh):
+        a = np.ascontiguousarray(xh, dtype=np.complex128)
+        out = np.empty(self.shape, np.float64)
+        _check(lib().qx_fft_c2r(self._h, _p(a.view(np.float64)), _p(out)))
+        return out
+
+    # -- initial conditions -------------------------------------------------------------
+    def ic_solitons(self, solitons, bkg=1.0):
+        t = np.ascontiguousarray(np.asarray(solitons, np.float64).reshape(-1, 5))
+        _check(lib().qx_ic_solitons(self._h, _p(t), t.shape[0], float(bkg)))
+
+    def ic_grf(self, seed, amp=0.1):
+        _check(lib().qx_ic_grf(self._h, int(seed), float(amp)))
+
+    def ic_blobs(self, table, bkg=0.1):
+        t = np.ascontiguousarray(np.asarray(table, np.float64).reshape(-1, 6))
+        _check(lib().qx_ic_blobs(self._h, _p(t), t.shape[0], float(bkg)))
+
+    # -- plumbing ---------------------------------------------------------------------------
+    @property
+    def stream(self):
+        return lib().qx_stream(self._h)
+
+    def synchronize(self):
+        _check(lib().qx_synchronize(self._h))
+
+
+def blob_table(nblobs, seed, L):
+    """Blob parameters (rho_c, r_c, x, y, z, phi) drawn from the keyed RNG:
+    rho_c ~ U[1,10], r_c ~ U[0.3,1], centre ~ U[0,L)^3, phi ~ U[0,2pi)."""
+    from .rng import uniform
+    t = np.zeros((nblobs, 6))
+    for j in range(nblobs):
+        st = 16 + j
+        t[j] = [1.0 + 9.0 * uniform(seed, st, 0), 0.3 + 0.7 * uniform(seed, st, 1),
+                L * uniform(seed, st, 2), L * uniform(seed, st, 3), L * uniform(seed, st, 4),
+                2.0 * math.pi * uniform(seed, st, 5)]
+    return t

@@ -1,0 +1,96 @@
+// qx_engine.hpp -- the per-(N, precision) set of FFT pass launchers.  Each
+// precision is instantiated in its own translation unit (qx_lib_f32.hip,
+// qx_lib_f64.hip) so the two compile in parallel.
+#pragma once
+#include <memory>
+#include "qx_kernels.hpp"
+
+namespace qx {
+
+// ---------------------------------------------------------------------------
+// per-(N, precision) kernel set
+// ---------------------------------------------------------------------------
+struct Engine {
+    virtual ~Engine() = default;
+    virtual void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) = 0;
+    virtual void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
+                         hipStream_t s) = 0;
+    virtual void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows,
+                          double pref, double rho_bar, hipStream_t s) = 0;
+    virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, double kick_dt,
+                          int flags, unsigned long long *vmax, hipStream_t s) = 0;
+    void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
+};
+
+template <int N, typename R> struct EngineT final : Engine {
+    using C = typename cplx<R>::T;
+    using G = Geom<N, R>;
+    using P = Plan<N>;
+
+    void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) override
+    {
+        const unsigned blocks = (unsigned)((nlines + G::LPB - 1) / G::LPB);
+        C *d = (C *)data;
+        const C *t = (const C *)tw;
+        if (op < 0) contig_fft<N, R, -1><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        else if (op > 0) contig_fft<N, R, +1><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        else contig_fft<N, R, 0><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+    }
+    void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
+                 hipStream_t s) override
+    {
+        const long long ntiles = (inner + G::TC - 1) / G::TC;
+        const unsigned blocks = (unsigned)(outer * ntiles);
+        C *d = (C *)data;
+        const C *t = (const C *)tw;
+        if (op < 0) strided_fft<N, R, -1><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
+        else if (op > 0) strided_fft<N, R, +1><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
+        else strided_fft<N, R, 0><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
+    }
+    void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, double pref,
+                  double rho_bar, hipStream_t s) override
+    {
+        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
+        C *p = (C *)psi;
+        const R *vi = (const R *)vin;
+        C *o = (C *)vt;
+        const C *t = (const C *)tw;
+        if (mode == R2C_REAL)
+            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, (R)pref, (R)rho_bar, t);
+        else if (mode == R2C_DENSITY)
+            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, (R)pref, (R)rho_bar, t);
+        else
+            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, (R)pref, (R)rho_bar, t);
+    }
+    void c2r_rows(const void *vt, void *V, void *psi, long long rows, double kick_dt, int flags,
+                  unsigned long long *vmax, hipStream_t s) override
+    {
+        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
+        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows,
+                                                             (R)kick_dt, flags, vmax, (const C *)tw);
+    }
+};
+
+template <typename R> std::unique_ptr<Engine> make_engine_t(int N)
+{
+    switch (N) {
+    case 2: return std::make_unique<EngineT<2, R>>();
+    case 4: return std::make_unique<EngineT<4, R>>();
+    case 8: return std::make_unique<EngineT<8, R>>();
+    case 16: return std::make_unique<EngineT<16, R>>();
+    case 32: return std::make_unique<EngineT<32, R>>();
+    case 64: return std::make_unique<EngineT<64, R>>();
+    case 128: return std::make_unique<EngineT<128, R>>();
+    case 256: return std::make_unique<EngineT<256, R>>();
+    case 512: return std::make_unique<EngineT<512, R>>();
+    case 1024: return std::make_unique<EngineT<1024, R>>();
+    case 2048: return std::make_unique<EngineT<2048, R>>();
+    case 4096: return std::make_unique<EngineT<4096, R>>();
+    default: return nullptr;
+    }
+}
+
+std::unique_ptr<Engine> make_engine_f32(int N);
+std::unique_ptr<Engine> make_engine_f64(int N);
+
+}  // namespace qx

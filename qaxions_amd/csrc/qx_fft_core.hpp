@@ -1,0 +1,185 @@
+// qx_fft_core.hpp -- register/LDS Stockham FFT building blocks for gfx950.
+//
+// A line of N complex values is spread over TPL = N/E threads; thread t holds
+// the E values at positions t + q*TPL (q = 0..E-1).  Every Stockham stage
+// reads its inputs in that distribution and, for the last stage, also leaves
+// its outputs in it, so global loads and stores of a contiguous line are
+// coalesced and a forward transform can flow straight into a pointwise
+// multiply and an inverse transform without touching LDS in between.
+//
+// Stages (radix R, Ns = product of earlier radices) follow the self-sorting
+// Stockham formulation: butterfly j takes x[j + r*N/R], twiddles by
+// W_{Ns R}^{r (j mod Ns)}, and writes y[(j/Ns) Ns R + (j mod Ns) + r Ns].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qx {
+
+template <typename R> struct cplx;
+template <> struct cplx<float> { using T = float2; };
+template <> struct cplx<double> { using T = double2; };
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+
+// multiply by -i (DIR < 0) or +i (DIR > 0)
+template <int DIR, typename C> __device__ __forceinline__ C mul_i(C a)
+{
+    return DIR < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
+}
+
+// ---------------------------------------------------------------------------
+// small DFTs in registers; DIR < 0 is e^{-2 pi i jk/R} (forward)
+// ---------------------------------------------------------------------------
+template <int R, int DIR, typename C> struct Dft;
+
+template <int DIR, typename C> struct Dft<1, DIR, C> {
+    static __device__ __forceinline__ void run(C *) {}
+};
+template <int DIR, typename C> struct Dft<2, DIR, C> {
+    static __device__ __forceinline__ void run(C *a)
+    {
+        C t = a[1];
+        a[1] = csub(a[0], t);
+        a[0] = cadd(a[0], t);
+    }
+};
+template <int DIR, typename C> struct Dft<4, DIR, C> {
+    static __device__ __forceinline__ void run(C *a)
+    {
+        C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        C t2 = cadd(a[1], a[3]), t3 = mul_i<DIR>(csub(a[1], a[3]));
+        a[0] = cadd(t0, t2);
+        a[2] = csub(t0, t2);
+        a[1] = cadd(t1, t3);
+        a[3] = csub(t1, t3);
+    }
+};
+template <int DIR, typename C> struct Dft<8, DIR, C> {
+    static __device__ __forceinline__ void run(C *a)
+    {
+        using Rt = decltype(a[0].x);
+        const Rt h = (Rt)0.70710678118654752440084436210485;
+        C e[4] = {a[0], a[2], a[4], a[6]};
+        C o[4] = {a[1], a[3], a[5], a[7]};
+        Dft<4, DIR, C>::run(e);
+        Dft<4, DIR, C>::run(o);
+        // twiddle the odd half by W8^k
+        C o1, o3;
+        if (DIR < 0) {
+            o1 = C{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+            o3 = C{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+        } else {
+            o1 = C{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)};
+            o3 = C{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)};
+        }
+        C o2 = mul_i<DIR>(o[2]);
+        a[0] = cadd(e[0], o[0]);
+        a[4] = csub(e[0], o[0]);
+        a[1] = cadd(e[1], o1);
+        a[5] = csub(e[1], o1);
+        a[2] = cadd(e[2], o2);
+        a[6] = csub(e[2], o2);
+        a[3] = cadd(e[3], o3);
+        a[7] = csub(e[3], o3);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// compile-time plan: stage 0 has radix 2, 4 or 8, the rest radix 8
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <int N> struct Plan {
+    static constexpr int E = N < 8 ? N : 8;      // values per thread
+    static constexpr int TPL = N / E;            // threads per line
+    static constexpr int LG = ilog2(N);
+    static constexpr int NST = (LG + 2) / 3;     // number of stages
+    static constexpr int R0 = (LG % 3 == 0) ? 8 : ((LG % 3 == 1) ? 2 : 4);
+    static constexpr int radix(int s) { return N < 8 ? N : (s == 0 ? R0 : 8); }
+    static constexpr int ns(int s) { return s == 0 ? 1 : (N < 8 ? 1 : R0 * ipow8(s - 1)); }
+    static constexpr int ipow8(int k) { return k == 0 ? 1 : 8 * ipow8(k - 1); }
+    // padded LDS slots for one contiguous line (one pad slot per 8 values)
+    static constexpr int PADDED = N + N / 8 + 1;
+};
+
+// LDS addressing of line positions
+struct LdsContig {  // [line][pos] with one pad slot per 8 values
+    __device__ __forceinline__ static int at(int pos, int) { return pos + (pos >> 3); }
+};
+struct LdsCols {    // [pos][column], columns interleaved
+    __device__ __forceinline__ static int at(int pos, int tc) { return pos * tc; }
+};
+
+// One Stockham stage.  v holds x[t + q TPL]; on exit (not last) v holds the
+// next stage's inputs, read back from LDS; (last) v holds y[t + q TPL].
+template <int N, int S, int DIR, typename L, typename C>
+__device__ __forceinline__ void fft_stage(C (&v)[Plan<N>::E], int t, C *line, int tc, const C *tw)
+{
+    using P = Plan<N>;
+    constexpr int E = P::E, TPL = P::TPL;
+    constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
+    constexpr bool last = (S == P::NST - 1);
+    constexpr int TWS = N / (Ns * R);   // twiddle table stride
+    C out[E];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int j = t + m * TPL;
+        const int k = j & (Ns - 1);
+        C u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = v[m + r * NB];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C w = tw[r * k * TWS];
+                if (DIR > 0) w.y = -w.y;
+                u[r] = cmul(u[r], w);
+            }
+        }
+        Dft<R, DIR, C>::run(u);
+        if (last) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[m + r * NB] = u[r];
+        } else {
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[L::at(base + r * Ns, tc)] = u[r];
+        }
+    }
+    if (last) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = out[q];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = line[L::at(t + q * TPL, tc)];
+        __syncthreads();
+    }
+}
+
+template <int N, int S, int DIR, typename L, typename C>
+__device__ __forceinline__ void stage_loop(C (&v)[Plan<N>::E], int t, C *line, int tc, const C *tw)
+{
+    if constexpr (S < Plan<N>::NST) {
+        fft_stage<N, S, DIR, L>(v, t, line, tc, tw);
+        stage_loop<N, S + 1, DIR, L>(v, t, line, tc, tw);
+    }
+}
+
+// Full N-point transform of the line held in v.  All threads of the block
+// must call it (it contains block barriers).  `line` is this line's LDS
+// area, `tc` the column interleave for LdsCols, `tw` the LDS twiddle table
+// tw[m] = e^{-2 pi i m/N}.
+template <int N, int DIR, typename L, typename C>
+__device__ __forceinline__ void fft_line(C (&v)[Plan<N>::E], int t, C *line, int tc, const C *tw)
+{
+    stage_loop<N, 0, DIR, L>(v, t, line, tc, tw);
+}
+
+}  // namespace qx

@@ -1,0 +1,442 @@
+// qx_kernels.hpp -- HIP kernels of the KDK split-step path for gfx950.
+//
+// Layouts in HBM (d = 2 or 3, S = N^d sites, rows = S/N):
+//   psi  : S complex, row-major [i0][i1][i2], last index fastest (as the
+//          reference's fftw_complex array, src/field.h:92);
+//   V    : S real, same order (src/field.h:94);
+//   vt   : the half spectrum of the real potential, stored k-major:
+//          vt[k_last * rows + row] for k_last in [0, N/2] and row the
+//          flattened leading indices.  Putting k_last outermost makes every
+//          remaining transform a contiguous or N-strided pass and turns the
+//          real<->complex passes into row-tile transposes (the reference's
+//          own layout, [row][k_last], is produced only at the API boundary).
+//
+// Pass kinds:
+//   contig_fft   : lines along the contiguous axis, optional fused multiply
+//                  (forward, pointwise factor, inverse) for the 2D potential;
+//   strided_fft  : columns at stride `inner` (the leading axes), optional
+//                  fused drift or Poisson multiply;
+//   r2c_rows     : real rows -> vt, fused with the inverse transform of psi
+//                  and the density pref(|psi|^2 - rho_bar) (two real rows
+//                  packed in one complex transform);
+//   c2r_rows     : vt -> real rows, fused with max|V|, the kick
+//                  psi <- psi e^{-i dt V} and the forward transform of psi.
+#pragma once
+#include "qx_fft_core.hpp"
+
+namespace qx {
+
+// ---------------------------------------------------------------------------
+// launch geometry shared by host and device
+// ---------------------------------------------------------------------------
+template <int N, typename R> struct Geom {
+    using P = Plan<N>;
+    using C = typename cplx<R>::T;
+    // contig_fft: lines per block
+    static constexpr int LPB = (256 / P::TPL) > 0 ? (256 / P::TPL) : 1;
+    static constexpr int CONTIG_THREADS = LPB * P::TPL;
+    // strided_fft: columns per block (LDS budget 64 KiB, <= 1024 threads)
+    static constexpr int TC0 = (65536 / (N * (int)sizeof(C))) < 1 ? 1 : (65536 / (N * (int)sizeof(C)));
+    static constexpr int TC1 = TC0 > 16 ? 16 : TC0;
+    static constexpr int TC = (TC1 * P::TPL > 1024) ? (1024 / P::TPL) : TC1;
+    static constexpr int STRIDED_THREADS = TC * P::TPL;
+    // row kernels: pairs of rows per block
+    static constexpr int NP0 = (256 / P::TPL) > 8 ? (256 / P::TPL) : 8;
+    static constexpr int NP1 = (NP0 * P::TPL > 1024) ? (1024 / P::TPL) : NP0;
+    // LDS budget of ~80 KiB: per pair max(exchange line, two staging rows)
+    static constexpr int PAIR_SLOTS = P::PADDED > (N + 2) ? P::PADDED : (N + 2);
+    static constexpr int NP2 = (81920 - N * (int)sizeof(C)) / (PAIR_SLOTS * (int)sizeof(C));
+    static constexpr int NP = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
+    static constexpr int TL = 2 * NP;                 // rows per block
+    static constexpr int ROW_THREADS = NP * P::TPL;
+    static constexpr int XCH = NP * P::PADDED;        // exchange slots
+    static constexpr int STG = (N / 2 + 1) * TL;      // staging slots
+    static constexpr int ROW_LDS = XCH > STG ? XCH : STG;
+};
+
+// pointwise factor in fused passes
+enum { MUL_NONE = 0, MUL_DRIFT = 1, MUL_POISSON = 2 };
+// geometry of the multiplied pass: how the 3 (or 2) integer frequencies of
+// an element follow from (line position p, column/line index c, outer o)
+enum {
+    GEO_PSI3 = 0,  // psi, axis 0 strided, c = i1*N + i2 (already transformed)
+    GEO_PSI2 = 1,  // psi 2D, axis 0 strided, c = k1
+    GEO_VT3 = 2,   // vt 3D, axis i0 strided inside plane o = k2, c = k1
+    GEO_VT2 = 3,   // vt 2D, contiguous lines along i0, line index = k1
+};
+
+struct MulArgs {
+    int kind;        // MUL_*
+    int geo;         // GEO_*
+    double cphase;   // drift: -dt/2 (2 pi / L)^2
+    double inv_s;    // 1/S
+    double pfac;     // Poisson: 1/((2 pi / L)^2 S)
+    double k0fac;    // Poisson factor of the k = 0 mode
+};
+
+__device__ __forceinline__ int foldi(int i, int N) { return (i <= N / 2) ? i : i - N; }
+
+// e^{i cphase m} / S.  The argument is formed and reduced mod 2 pi in FP64
+// (Cody-Waite, two-word 2 pi) so that high-k phases keep full FP32 accuracy.
+template <typename R> struct Phase;
+template <> struct Phase<float> {
+    static __device__ __forceinline__ float2 factor(double cphase, int m, double inv_s)
+    {
+        const double th = cphase * (double)m;
+        const double kk = rint(th * 0.15915494309189533577);
+        double r = fma(-kk, 6.283185307179586232, th);
+        r = fma(-kk, 2.4492935982947064e-16, r);
+        float s, c;
+        sincosf((float)r, &s, &c);
+        const float f = (float)inv_s;
+        return float2{c * f, s * f};
+    }
+};
+template <> struct Phase<double> {
+    static __device__ __forceinline__ double2 factor(double cphase, int m, double inv_s)
+    {
+        const double th = cphase * (double)m;
+        const double kk = rint(th * 0.15915494309189533577);
+        double r = fma(-kk, 6.283185307179586232, th);
+        r = fma(-kk, 2.4492935982947064e-16, r);
+        double s, c;
+        sincos(r, &s, &c);
+        return double2{c * inv_s, s * inv_s};
+    }
+};
+
+__device__ __forceinline__ void sincos_r(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sincos(x, s, c); }
+
+template <typename R, typename C>
+__device__ __forceinline__ C apply_factor(const MulArgs &a, int m, C x)
+{
+    if (a.kind == MUL_DRIFT) {
+        C f = Phase<R>::factor(a.cphase, m, a.inv_s);
+        return cmul(x, f);
+    }
+    R f = (m == 0) ? (R)a.k0fac : -(R)a.pfac / (R)m;
+    return C{x.x * f, x.y * f};
+}
+
+template <int N, typename C>
+__device__ __forceinline__ void load_tw(C *tw, const C *twg)
+{
+    for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+}
+
+// ---------------------------------------------------------------------------
+// contiguous lines: data[line*N + pos]
+// ---------------------------------------------------------------------------
+template <int N, typename R, int OP>  // OP: -1 forward, +1 inverse, 0 fwd*mul*inv
+__global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
+contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul,
+           const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    using G = Geom<N, R>;
+    __shared__ C lds[G::LPB * P::PADDED];
+    __shared__ C tw[N];
+    load_tw<N>(tw, twg);
+    const int t = threadIdx.x % P::TPL, l = threadIdx.x / P::TPL;
+    const long long line = (long long)blockIdx.x * G::LPB + l;
+    const bool ok = line < nlines;
+    C *mine = lds + l * P::PADDED;
+    C v[P::E];
+    C *src = data + line * N;
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) v[q] = ok ? src[t + q * P::TPL] : C{0, 0};
+    __syncthreads();
+    if (OP < 0) {
+        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+    } else if (OP > 0) {
+        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+    } else {
+        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+        // GEO_VT2: position = k0 (folded), line = k1 (half axis, unfolded)
+        const int k1 = (int)line;
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int n0 = foldi(t + q * P::TPL, N);
+            v[q] = apply_factor<R>(mul, n0 * n0 + k1 * k1, v[q]);
+        }
+        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+    }
+    if (ok) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) src[t + q * P::TPL] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strided columns: data[o*N*inner + p*inner + col], col in [0, inner)
+// ---------------------------------------------------------------------------
+template <int N, typename R, int OP>  // OP: -1 forward, +1 inverse, 0 fwd*mul*inv
+__global__ void __launch_bounds__((Geom<N, R>::STRIDED_THREADS))
+strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
+            MulArgs mul, const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    using G = Geom<N, R>;
+    constexpr int TC = G::TC;
+    __shared__ C lds[N * TC];
+    __shared__ C tw[N];
+    load_tw<N>(tw, twg);
+    const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+    const long long o = blockIdx.x / ntiles;
+    const long long col = (blockIdx.x % ntiles) * TC + c;
+    const bool ok = col < inner;
+    C *base = data + o * (long long)N * inner + col;
+    C v[P::E];
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) v[q] = ok ? base[(long long)(t + q * P::TPL) * inner] : C{0, 0};
+    __syncthreads();
+    if (OP < 0) {
+        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+    } else if (OP > 0) {
+        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+    } else {
+        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+        int mc;
+        if (mul.geo == GEO_PSI3) {
+            const int a = foldi((int)(col / N), N), b = foldi((int)(col % N), N);
+            mc = a * a + b * b;
+        } else if (mul.geo == GEO_PSI2) {
+            const int a = foldi((int)col, N);
+            mc = a * a;
+        } else {  // GEO_VT3
+            const int a = foldi((int)col, N);
+            mc = a * a + (int)(o * o);
+        }
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int n0 = foldi(t + q * P::TPL, N);
+            v[q] = apply_factor<R>(mul, n0 * n0 + mc, v[q]);
+        }
+        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+    }
+    if (ok) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) base[(long long)(t + q * P::TPL) * inner] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// real rows -> vt (half spectrum, k-major), two rows per complex transform
+// ---------------------------------------------------------------------------
+enum {
+    R2C_REAL = 0,      // input: real rows (vin)
+    R2C_DENSITY = 1,   // input: psi rows, rho = pref (|psi|^2 - rho_bar)
+    R2C_INV_DENSITY = 2,  // input: psi rows transformed along the last axis;
+                          // inverse-transform, store psi, then as DENSITY
+};
+
+template <int N, typename R, int MODE>
+__global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
+r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
+         typename cplx<R>::T *__restrict__ vt, long long rows, R pref, R rho_bar,
+         const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    using G = Geom<N, R>;
+    constexpr int TL = G::TL, H1 = N / 2 + 1;
+    __shared__ C lds[G::ROW_LDS];
+    __shared__ C tw[N];
+    load_tw<N>(tw, twg);
+    const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
+    const long long r0 = (long long)blockIdx.x * TL;
+    const long long ra = r0 + 2 * p, rb = ra + 1;
+    const bool oka = ra < rows, okb = rb < rows;
+    C *mine = lds + p * P::PADDED;
+    R xa[P::E], xb[P::E];
+    if (MODE == R2C_REAL) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            xa[q] = oka ? vin[ra * N + pos] : (R)0;
+            xb[q] = okb ? vin[rb * N + pos] : (R)0;
+        }
+        __syncthreads();
+    } else {
+        C a[P::E], b[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+        }
+        __syncthreads();
+        if (MODE == R2C_INV_DENSITY) {
+            fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
+            fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                if (oka) psi[ra * N + pos] = a[q];
+                if (okb) psi[rb * N + pos] = b[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            xa[q] = pref * (a[q].x * a[q].x + a[q].y * a[q].y - rho_bar);
+            xb[q] = pref * (b[q].x * b[q].x + b[q].y * b[q].y - rho_bar);
+        }
+    }
+    // z = xa + i xb, Z = FFT(z); Xa[k] = (Z[k] + conj Z[N-k]) / 2,
+    // Xb[k] = (Z[k] - conj Z[N-k]) / (2i)
+    C z[P::E];
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) z[q] = C{xa[q], xb[q]};
+    fft_line<N, -1, LdsContig>(z, t, mine, 1, tw);
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = z[q];
+    __syncthreads();
+    C oa[P::E / 2 + 1], ob[P::E / 2 + 1];
+#pragma unroll
+    for (int q = 0; q <= P::E / 2; ++q) {
+        const int k = t + q * P::TPL;
+        if (k <= N / 2) {
+            const C zk = mine[LdsContig::at(k, 1)];
+            const C zm = mine[LdsContig::at((N - k) & (N - 1), 1)];
+            oa[q] = C{(R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y)};
+            ob[q] = C{(R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x)};
+        }
+    }
+    __syncthreads();
+    C *stg = lds;  // staging aliases the exchange area: stg[k*TL + l]
+#pragma unroll
+    for (int q = 0; q <= P::E / 2; ++q) {
+        const int k = t + q * P::TPL;
+        if (k <= N / 2) {
+            stg[k * TL + 2 * p] = oa[q];
+            stg[k * TL + 2 * p + 1] = ob[q];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+        const int k = i / TL, l = i % TL;
+        if (r0 + l < rows) vt[(long long)k * rows + r0 + l] = stg[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vt (or real V) -> real rows, fused max|V|, kick and forward transform
+// ---------------------------------------------------------------------------
+enum {
+    C2R_SRC_VT = 1,     // rows come from vt (else from the real array V)
+    C2R_STORE_V = 2,    // write the real rows to V
+    C2R_VMAX = 4,       // fold max|V| into *vmax_bits
+    C2R_KICK = 8,       // psi <- psi exp(-i kick_dt V)
+    C2R_FWD = 16,       // forward-transform psi along the last axis
+};
+
+template <int N, typename R>
+__global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
+c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
+         typename cplx<R>::T *__restrict__ psi, long long rows, R kick_dt, int flags,
+         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    using G = Geom<N, R>;
+    constexpr int TL = G::TL, H1 = N / 2 + 1;
+    __shared__ C lds[G::ROW_LDS];
+    __shared__ C tw[N];
+    __shared__ double wmax[16];
+    load_tw<N>(tw, twg);
+    const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
+    const long long r0 = (long long)blockIdx.x * TL;
+    const long long ra = r0 + 2 * p, rb = ra + 1;
+    const bool oka = ra < rows, okb = rb < rows;
+    C *mine = lds + p * P::PADDED;
+    R va[P::E], vb[P::E];
+    if (flags & C2R_SRC_VT) {
+        C *stg = lds;
+        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+            const int k = i / TL, l = i % TL;
+            stg[i] = (r0 + l < rows) ? vt[(long long)k * rows + r0 + l] : C{0, 0};
+        }
+        __syncthreads();
+        C z[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int j = t + q * P::TPL;
+            const bool lo = j <= N / 2;
+            const int k = lo ? j : N - j;
+            C xa = stg[k * TL + 2 * p], xb = stg[k * TL + 2 * p + 1];
+            if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
+            // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
+            z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
+        }
+        __syncthreads();
+        fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
+    } else {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            va[q] = oka ? V[ra * N + pos] : (R)0;
+            vb[q] = okb ? V[rb * N + pos] : (R)0;
+        }
+    }
+    if (flags & C2R_STORE_V) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            if (oka) V[ra * N + pos] = va[q];
+            if (okb) V[rb * N + pos] = vb[q];
+        }
+    }
+    if (flags & C2R_VMAX) {
+        double m = 0.0;
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            if (oka) m = fmax(m, (double)fabs(va[q]));
+            if (okb) m = fmax(m, (double)fabs(vb[q]));
+        }
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) m = fmax(m, wmax[w]);
+            // non-negative doubles order like their bit patterns
+            atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+    if (flags & (C2R_KICK | C2R_FWD)) {
+        C a[P::E], b[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+        }
+        if (flags & C2R_KICK) {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                R sa, ca, sb, cb;
+                sincos_r(-kick_dt * va[q], &sa, &ca);
+                sincos_r(-kick_dt * vb[q], &sb, &cb);
+                a[q] = C{a[q].x * ca - a[q].y * sa, a[q].x * sa + a[q].y * ca};
+                b[q] = C{b[q].x * cb - b[q].y * sb, b[q].x * sb + b[q].y * cb};
+            }
+        }
+        if (flags & C2R_FWD) {
+            __syncthreads();
+            fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
+            fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
+        }
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            if (oka) psi[ra * N + pos] = a[q];
+            if (okb) psi[rb * N + pos] = b[q];
+        }
+    }
+}
+
+}  // namespace qx

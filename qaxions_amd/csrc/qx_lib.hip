@@ -1,0 +1,825 @@
+// qx_lib.hip -- host side of libqx: contexts, pass schedules, C ABI.
+//
+// The step schedule mirrors the reference time loop (src/main.cxx:65-85)
+// and Field's operators (src/field.cxx:93-402); the fused schedule of
+// qx_leapfrog merges neighbouring operators into shared HBM passes:
+//
+//   3D step:  c2r_rows[vt -> V, kick, fwd axis 2]      (K + start of D)
+//             strided fwd axis 1
+//             strided axis 0: fwd, e^{-i dt k^2/2}/S, inv   (D)
+//             strided inv axis 1
+//             r2c_rows[inv axis 2, store psi, density, r2c] (end of D, P)
+//             contig fwd / strided Poisson / contig inv on vt (P)
+//   2D step:  c2r_rows / strided drift / r2c_rows / contig Poisson.
+#include "../../include/qx.h"
+#include "qx_aux.hpp"
+#include "qx_engine.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace qx;
+
+namespace {
+
+thread_local std::string g_err;
+
+qx_status fail(qx_status s, const std::string &msg)
+{
+    g_err = msg;
+    return s;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(QX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+
+std::unique_ptr<Engine> make_engine(int N, bool fp64) { return fp64 ? make_engine_f64(N) : make_engine_f32(N); }
+
+unsigned grid_for(long long n, int threads = 256, long long cap = 4096)
+{
+    long long b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct qx_ctx {
+    int dim = 0, N = 0, device = 0;
+    double L = 0.0;
+    qx_precision prec = QX_FP32;
+    long long S = 0, rows = 0, H1 = 0, H = 0;
+    size_t csz = 0, rsz = 0;
+    void *psi = nullptr, *V = nullptr, *vt = nullptr;
+    void *aux_c = nullptr;   // S complex scratch (lazy)
+    void *aux_vt = nullptr;  // H complex scratch (lazy)
+    void *aux_r = nullptr;   // S real scratch (lazy)
+    void *stage = nullptr;   // complex128 staging for host transfers (lazy)
+    double *partials = nullptr;      // device reduction partials
+    double *h_partials = nullptr;    // pinned host copy
+    unsigned long long *vmax_bits = nullptr;
+    double *spec_P = nullptr;
+    unsigned long long *spec_cnt = nullptr;
+    int spec_cap = 0;
+    hipStream_t stream = nullptr;
+    qx_physics ph{4.0 * M_PI, 1.0, 1, 0};
+    double vmax = 0.0;
+    std::unique_ptr<Engine> eng;
+};
+
+namespace {
+
+constexpr long long STAGE_ELEMS = 1ll << 22;  // complex128 elements per transfer chunk
+constexpr int MAX_PART = 1024;
+
+qx_status ensure(qx_ctx *c, void **p, size_t bytes)
+{
+    if (*p) return QX_OK;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(QX_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    (void)c;
+    return QX_OK;
+}
+
+#define TRY(x)                          \
+    do {                                \
+        qx_status s_ = (x);             \
+        if (s_ != QX_OK) return s_;     \
+    } while (0)
+
+qx_status sync(qx_ctx *c)
+{
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QX_OK;
+}
+
+bool fp64(const qx_ctx *c) { return c->prec == QX_FP64; }
+
+// complex128 host -> device state (converted in chunks through `stage`)
+qx_status upload_c(qx_ctx *c, const double *h, void *d, long long n)
+{
+    if (fp64(c)) {
+        HIPCHK(hipMemcpyAsync(d, h, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+        return sync(c);
+    }
+    TRY(ensure(c, &c->stage, (size_t)STAGE_ELEMS * 16));
+    for (long long off = 0; off < n; off += STAGE_ELEMS) {
+        const long long m = std::min(STAGE_ELEMS, n - off);
+        HIPCHK(hipMemcpyAsync(c->stage, h + 2 * off, (size_t)m * 16, hipMemcpyHostToDevice, c->stream));
+        c128_to_c<float><<<grid_for(m), 256, 0, c->stream>>>((const double2 *)c->stage,
+                                                            (float2 *)d + off, m);
+    }
+    return sync(c);
+}
+qx_status download_c(qx_ctx *c, const void *d, double *h, long long n)
+{
+    if (fp64(c)) {
+        HIPCHK(hipMemcpyAsync(h, d, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+        return sync(c);
+    }
+    TRY(ensure(c, &c->stage, (size_t)STAGE_ELEMS * 16));
+    for (long long off = 0; off < n; off += STAGE_ELEMS) {
+        const long long m = std::min(STAGE_ELEMS, n - off);
+        c_to_c128<float><<<grid_for(m), 256, 0, c->stream>>>((const float2 *)d + off,
+                                                            (double2 *)c->stage, m);
+        HIPCHK(hipMemcpyAsync(h + 2 * off, c->stage, (size_t)m * 16, hipMemcpyDeviceToHost, c->stream));
+        TRY(sync(c));
+    }
+    return sync(c);
+}
+qx_status upload_r(qx_ctx *c, const double *h, void *d, long long n)
+{
+    if (fp64(c)) {
+        HIPCHK(hipMemcpyAsync(d, h, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+        return sync(c);
+    }
+    TRY(ensure(c, &c->stage, (size_t)STAGE_ELEMS * 16));
+    const long long chunk = 2 * STAGE_ELEMS;
+    for (long long off = 0; off < n; off += chunk) {
+        const long long m = std::min(chunk, n - off);
+        HIPCHK(hipMemcpyAsync(c->stage, h + off, (size_t)m * 8, hipMemcpyHostToDevice, c->stream));
+        f64_to_r<float><<<grid_for(m), 256, 0, c->stream>>>((const double *)c->stage, (float *)d + off, m);
+    }
+    return sync(c);
+}
+qx_status download_r(qx_ctx *c, const void *d, double *h, long long n)
+{
+    if (fp64(c)) {
+        HIPCHK(hipMemcpyAsync(h, d, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+        return sync(c);
+    }
+    TRY(ensure(c, &c->stage, (size_t)STAGE_ELEMS * 16));
+    const long long chunk = 2 * STAGE_ELEMS;
+    for (long long off = 0; off < n; off += chunk) {
+        const long long m = std::min(chunk, n - off);
+        r_to_f64<float><<<grid_for(m), 256, 0, c->stream>>>((const float *)d + off, (double *)c->stage, m);
+        HIPCHK(hipMemcpyAsync(h + off, c->stage, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
+        TRY(sync(c));
+    }
+    return sync(c);
+}
+
+MulArgs drift_args(const qx_ctx *c, double dt)
+{
+    const double dk = TWO_PI / c->L;
+    MulArgs m{};
+    m.kind = MUL_DRIFT;
+    m.geo = c->dim == 3 ? GEO_PSI3 : GEO_PSI2;
+    m.cphase = -0.5 * dt * dk * dk;
+    m.inv_s = 1.0 / (double)c->S;
+    return m;
+}
+MulArgs poisson_args(const qx_ctx *c)
+{
+    const double dk = TWO_PI / c->L;
+    MulArgs m{};
+    m.kind = MUL_POISSON;
+    m.geo = c->dim == 3 ? GEO_VT3 : GEO_VT2;
+    m.pfac = 1.0 / (dk * dk * (double)c->S);
+    m.k0fac = (c->ph.k0_mode == 1 && c->ph.rho_bar_mode == 0) ? -1.0 / (double)c->S : 0.0;
+    return m;
+}
+
+// ---- pass schedules ------------------------------------------------------
+void c2c_passes(qx_ctx *c, void *data, int sign)
+{
+    MulArgs none{};
+    const int N = c->N;
+    c->eng->contig(data, c->rows, sign, none, c->stream);
+    if (c->dim == 3) {
+        c->eng->strided(data, N, N, sign, none, c->stream);
+        c->eng->strided(data, 1, (long long)N * N, sign, none, c->stream);
+    } else {
+        c->eng->strided(data, 1, N, sign, none, c->stream);
+    }
+}
+// the two (3D) or one (2D) leading-axis passes of psi around the drift
+void drift_core(qx_ctx *c, double dt)
+{
+    MulArgs none{};
+    const int N = c->N;
+    const MulArgs m = drift_args(c, dt);
+    if (c->dim == 3) {
+        c->eng->strided(c->psi, N, N, -1, none, c->stream);
+        c->eng->strided(c->psi, 1, (long long)N * N, 0, m, c->stream);
+        c->eng->strided(c->psi, N, N, +1, none, c->stream);
+    } else {
+        c->eng->strided(c->psi, 1, N, 0, m, c->stream);
+    }
+}
+void vt_forward(qx_ctx *c, void *vt)
+{
+    MulArgs none{};
+    const int N = c->N;
+    if (c->dim == 3) {
+        c->eng->contig(vt, c->H1 * N, -1, none, c->stream);
+        c->eng->strided(vt, c->H1, N, -1, none, c->stream);
+    } else {
+        c->eng->contig(vt, c->H1, -1, none, c->stream);
+    }
+}
+void vt_inverse(qx_ctx *c, void *vt)
+{
+    MulArgs none{};
+    const int N = c->N;
+    if (c->dim == 3) {
+        c->eng->strided(vt, c->H1, N, +1, none, c->stream);
+        c->eng->contig(vt, c->H1 * N, +1, none, c->stream);
+    } else {
+        c->eng->contig(vt, c->H1, +1, none, c->stream);
+    }
+}
+void vt_poisson(qx_ctx *c, void *vt)
+{
+    MulArgs none{};
+    const int N = c->N;
+    const MulArgs m = poisson_args(c);
+    if (c->dim == 3) {
+        c->eng->contig(vt, c->H1 * N, -1, none, c->stream);
+        c->eng->strided(vt, c->H1, N, 0, m, c->stream);
+        c->eng->contig(vt, c->H1 * N, +1, none, c->stream);
+    } else {
+        c->eng->contig(vt, c->H1, 0, m, c->stream);
+    }
+}
+
+double rho_bar_eff(const qx_ctx *c) { return c->ph.rho_bar; }
+
+qx_status reset_vmax(qx_ctx *c)
+{
+    HIPCHK(hipMemsetAsync(c->vmax_bits, 0, sizeof(unsigned long long), c->stream));
+    return QX_OK;
+}
+qx_status read_vmax(qx_ctx *c)
+{
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, c->vmax_bits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    c->vmax = v;
+    return QX_OK;
+}
+
+// P pass of the reference: density, r2c, Poisson, c2r, max|V|
+qx_status potential_from_psi(qx_ctx *c)
+{
+    TRY(reset_vmax(c));
+    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->ph.pref, rho_bar_eff(c),
+                     c->stream);
+    vt_poisson(c, c->vt);
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, 0.0, C2R_SRC_VT | C2R_STORE_V | C2R_VMAX,
+                     c->vmax_bits, c->stream);
+    return read_vmax(c);
+}
+
+template <typename R, int KIND>
+qx_status reduce_t(qx_ctx *c, const void *psi, const void *V, long long n, double *sum, double *mx)
+{
+    const unsigned blocks = grid_for(n, RED_THREADS, MAX_PART);
+    reduce_kernel<R, KIND><<<blocks, RED_THREADS, 0, c->stream>>>(
+        (const typename cplx<R>::T *)psi, (const R *)V, n, c->N, c->dim, c->partials);
+    HIPCHK(hipMemcpyAsync(c->h_partials, c->partials, sizeof(double) * 2 * blocks,
+                          hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    double s = 0.0, m = 0.0;
+    for (unsigned b = 0; b < blocks; ++b) {
+        s += c->h_partials[2 * b];
+        m = std::max(m, c->h_partials[2 * b + 1]);
+    }
+    if (sum) *sum = s;
+    if (mx) *mx = m;
+    return QX_OK;
+}
+template <int KIND>
+qx_status reduce(qx_ctx *c, const void *psi, const void *V, long long n, double *sum, double *mx)
+{
+    return fp64(c) ? reduce_t<double, KIND>(c, psi, V, n, sum, mx)
+                   : reduce_t<float, KIND>(c, psi, V, n, sum, mx);
+}
+
+bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
+
+#define CHECK_CTX(c)                                                  \
+    do {                                                              \
+        if (!valid_ctx(c)) return fail(QX_EINVAL, "null context");    \
+        HIPCHK(hipSetDevice((c)->device));                            \
+    } while (0)
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *qx_last_error(void) { return g_err.c_str(); }
+const char *qx_version(void) { return "qx 0.1 gfx950"; }
+
+qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_ctx **out)
+{
+    if (!out) return fail(QX_EINVAL, "out is null");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(QX_EINVAL, "Dimension must be 2 or 3");
+    if (N < 2 || (N & (N - 1)) != 0 || N > (dim == 3 ? 1024 : 4096))
+        return fail(QX_EINVAL, "N must be a power of two in [2, 4096] (2D) or [2, 1024] (3D)");
+    if (!(L > 0.0)) return fail(QX_EINVAL, "L must be positive");
+    if (prec != QX_FP32 && prec != QX_FP64) return fail(QX_EINVAL, "unknown precision");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(QX_EINVAL, "no such device");
+    HIPCHK(hipSetDevice(device));
+
+    std::unique_ptr<qx_ctx> c(new qx_ctx());
+    c->dim = dim;
+    c->N = N;
+    c->L = L;
+    c->prec = prec;
+    c->device = device;
+    c->S = 1;
+    for (int d = 0; d < dim; ++d) c->S *= N;
+    c->rows = c->S / N;
+    c->H1 = N / 2 + 1;
+    c->H = c->rows * c->H1;
+    c->csz = prec == QX_FP64 ? 16 : 8;
+    c->rsz = prec == QX_FP64 ? 8 : 4;
+    c->eng = make_engine(N, prec == QX_FP64);
+    if (!c->eng) return fail(QX_EINVAL, "unsupported N");
+
+    auto cleanup = [&](qx_status s) {
+        qx_destroy(c.release());
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(QX_EDEVICE, "stream creation failed"));
+    qx_status s;
+    if ((s = ensure(c.get(), &c->psi, (size_t)c->S * c->csz)) != QX_OK) return cleanup(s);
+    if ((s = ensure(c.get(), &c->V, (size_t)c->S * c->rsz)) != QX_OK) return cleanup(s);
+    if ((s = ensure(c.get(), &c->vt, (size_t)c->H * c->csz)) != QX_OK) return cleanup(s);
+    if ((s = ensure(c.get(), &c->eng->tw, (size_t)N * c->csz)) != QX_OK) return cleanup(s);
+    if ((s = ensure(c.get(), (void **)&c->partials, sizeof(double) * 2 * MAX_PART)) != QX_OK)
+        return cleanup(s);
+    if ((s = ensure(c.get(), (void **)&c->vmax_bits, sizeof(unsigned long long))) != QX_OK)
+        return cleanup(s);
+    if (hipHostMalloc((void **)&c->h_partials, sizeof(double) * 2 * MAX_PART) != hipSuccess)
+        return cleanup(fail(QX_ENOMEM, "pinned host allocation failed"));
+
+    // twiddles in FP64, rounded once to the state precision
+    std::vector<double> tw(2 * (size_t)N);
+    for (int m = 0; m < N; ++m) {
+        double a = TWO_PI * (double)m / (double)N;
+        double cr = std::cos(a), ci = -std::sin(a);
+        if (4 * m == N) { cr = 0.0; ci = -1.0; }
+        if (2 * m == N) { cr = -1.0; ci = 0.0; }
+        if (4 * m == 3 * N) { cr = 0.0; ci = 1.0; }
+        tw[2 * m] = cr;
+        tw[2 * m + 1] = ci;
+    }
+    if (prec == QX_FP64) {
+        if (hipMemcpy(c->eng->tw, tw.data(), 16 * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(QX_EDEVICE, "twiddle upload failed"));
+    } else {
+        std::vector<float> tf(tw.begin(), tw.end());
+        if (hipMemcpy(c->eng->tw, tf.data(), 8 * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(QX_EDEVICE, "twiddle upload failed"));
+    }
+    if (hipMemsetAsync(c->psi, 0, (size_t)c->S * c->csz, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->V, 0, (size_t)c->S * c->rsz, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->vt, 0, (size_t)c->H * c->csz, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return cleanup(fail(QX_EDEVICE, "state initialisation failed"));
+    *out = c.release();
+    return QX_OK;
+}
+
+qx_status qx_destroy(qx_ctx *c)
+{
+    if (!c) return QX_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->psi, c->V, c->vt, c->aux_c, c->aux_vt, c->aux_r, c->stage,
+                    c->partials, c->vmax_bits, c->spec_P, c->spec_cnt,
+                    c->eng ? c->eng->tw : nullptr};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->h_partials) (void)hipHostFree(c->h_partials);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return QX_OK;
+}
+
+qx_status qx_set_physics(qx_ctx *c, const qx_physics *ph)
+{
+    if (!valid_ctx(c) || !ph) return fail(QX_EINVAL, "null argument");
+    if (ph->k0_mode != 0 && ph->k0_mode != 1) return fail(QX_EINVAL, "k0_mode must be 0 or 1");
+    if (ph->rho_bar_mode != 0 && ph->rho_bar_mode != 1)
+        return fail(QX_EINVAL, "rho_bar_mode must be 0 or 1");
+    c->ph = *ph;
+    return QX_OK;
+}
+qx_status qx_get_physics(const qx_ctx *c, qx_physics *ph)
+{
+    if (!valid_ctx(c) || !ph) return fail(QX_EINVAL, "null argument");
+    *ph = c->ph;
+    return QX_OK;
+}
+
+qx_status qx_upload_psi(qx_ctx *c, const double *h)
+{
+    CHECK_CTX(c);
+    if (!h) return fail(QX_EINVAL, "null psi");
+    return upload_c(c, h, c->psi, c->S);
+}
+qx_status qx_download_psi(qx_ctx *c, double *h)
+{
+    CHECK_CTX(c);
+    if (!h) return fail(QX_EINVAL, "null psi");
+    return download_c(c, c->psi, h, c->S);
+}
+qx_status qx_upload_V(qx_ctx *c, const double *h)
+{
+    CHECK_CTX(c);
+    if (!h) return fail(QX_EINVAL, "null V");
+    return upload_r(c, h, c->V, c->S);
+}
+qx_status qx_download_V(qx_ctx *c, double *h)
+{
+    CHECK_CTX(c);
+    if (!h) return fail(QX_EINVAL, "null V");
+    return download_r(c, c->V, h, c->S);
+}
+qx_status qx_device_state(qx_ctx *c, void **psi, void **V)
+{
+    if (!valid_ctx(c)) return fail(QX_EINVAL, "null context");
+    if (psi) *psi = c->psi;
+    if (V) *V = c->V;
+    return QX_OK;
+}
+
+
+// ---- split-step operators ---------------------------------------------------
+
+qx_status qx_kick(qx_ctx *c, double dt)
+{
+    CHECK_CTX(c);
+    if (fp64(c))
+        kick_kernel<double><<<grid_for(c->S), 256, 0, c->stream>>>((double2 *)c->psi, (const double *)c->V, dt, c->S);
+    else
+        kick_kernel<float><<<grid_for(c->S), 256, 0, c->stream>>>((float2 *)c->psi, (const float *)c->V, (float)dt, c->S);
+    return sync(c);
+}
+
+qx_status qx_drift(qx_ctx *c, double dt)
+{
+    CHECK_CTX(c);
+    MulArgs none{};
+    c->eng->contig(c->psi, c->rows, -1, none, c->stream);
+    drift_core(c, dt);
+    c->eng->contig(c->psi, c->rows, +1, none, c->stream);
+    return sync(c);
+}
+
+qx_status qx_update_potential(qx_ctx *c, double *vmax)
+{
+    CHECK_CTX(c);
+    TRY(potential_from_psi(c));
+    if (vmax) *vmax = c->vmax;
+    return QX_OK;
+}
+
+qx_status qx_poisson_half(qx_ctx *c, double *vh)
+{
+    CHECK_CTX(c);
+    if (!vh) return fail(QX_EINVAL, "null half spectrum");
+    TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
+    TRY(upload_c(c, vh, c->aux_vt, c->H));
+    const MulArgs m = poisson_args(c);
+    if (fp64(c))
+        poisson_ref_kernel<double><<<grid_for(c->H), 256, 0, c->stream>>>((double2 *)c->aux_vt, c->rows, c->N, c->dim, m.pfac, m.k0fac);
+    else
+        poisson_ref_kernel<float><<<grid_for(c->H), 256, 0, c->stream>>>((float2 *)c->aux_vt, c->rows, c->N, c->dim, m.pfac, m.k0fac);
+    TRY(sync(c));
+    return download_c(c, c->aux_vt, vh, c->H);
+}
+
+qx_status qx_compute_energy(qx_ctx *c, double *rhomax_over_mean)
+{
+    CHECK_CTX(c);
+    if (fp64(c))
+        density_kernel<double><<<grid_for(c->S), 256, 0, c->stream>>>((const double2 *)c->psi, (double *)c->V, c->S);
+    else
+        density_kernel<float><<<grid_for(c->S), 256, 0, c->stream>>>((const float2 *)c->psi, (float *)c->V, c->S);
+    double sum = 0.0, mx = 0.0;
+    TRY(reduce<RED_VABS>(c, nullptr, c->V, c->S, &sum, &mx));
+    if (rhomax_over_mean) *rhomax_over_mean = mx / (sum / (double)c->S);
+    return QX_OK;
+}
+
+qx_status qx_density_stats(qx_ctx *c, double *rhomax_over_mean, double *mass)
+{
+    CHECK_CTX(c);
+    double sum = 0.0, mx = 0.0;
+    TRY(reduce<RED_RHO>(c, c->psi, nullptr, c->S, &sum, &mx));
+    if (rhomax_over_mean) *rhomax_over_mean = mx / (sum / (double)c->S);
+    if (mass) *mass = std::pow(c->L / c->N, c->dim) * sum;
+    return QX_OK;
+}
+
+qx_status qx_energy(qx_ctx *c, double *mass, double *kinetic, double *potential)
+{
+    CHECK_CTX(c);
+    const double dV = std::pow(c->L / c->N, c->dim), dk = TWO_PI / c->L;
+    double sum = 0.0;
+    if (mass) {
+        TRY(reduce<RED_RHO>(c, c->psi, nullptr, c->S, &sum, nullptr));
+        *mass = dV * sum;
+    }
+    if (potential) {
+        TRY(reduce<RED_VRHO>(c, c->psi, c->V, c->S, &sum, nullptr));
+        *potential = 0.5 * dV * sum;
+    }
+    if (kinetic) {
+        TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+        HIPCHK(hipMemcpyAsync(c->aux_c, c->psi, (size_t)c->S * c->csz, hipMemcpyDeviceToDevice, c->stream));
+        c2c_passes(c, c->aux_c, -1);
+        TRY(reduce<RED_K2>(c, c->aux_c, nullptr, c->S, &sum, nullptr));
+        *kinetic = 0.5 * dV / (double)c->S * dk * dk * sum;
+    }
+    return QX_OK;
+}
+
+double qx_vmax(const qx_ctx *c) { return c ? c->vmax : 0.0; }
+
+qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int64_t *count,
+                      double *kmodes)
+{
+    CHECK_CTX(c);
+    if (source != 0 && source != 1) return fail(QX_EINVAL, "source must be 0 (V) or 1 (|psi|^2)");
+    if (nbins < 1 || spb < 1 || nbins > c->N / 2 + 1)
+        return fail(QX_EINVAL, "nbins must be in [1, N/2+1] and spb >= 1");
+    if (!Pk || !count || !kmodes) return fail(QX_EINVAL, "null output");
+    TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
+    if (c->spec_cap < nbins) {
+        if (c->spec_P) (void)hipFree(c->spec_P);
+        if (c->spec_cnt) (void)hipFree(c->spec_cnt);
+        c->spec_P = nullptr;
+        c->spec_cnt = nullptr;
+        c->spec_cap = 0;
+        TRY(ensure(c, (void **)&c->spec_P, sizeof(double) * (c->N / 2 + 1)));
+        TRY(ensure(c, (void **)&c->spec_cnt, sizeof(unsigned long long) * (c->N / 2 + 1)));
+        c->spec_cap = c->N / 2 + 1;
+    }
+    if (source == 0)
+        c->eng->r2c_rows(R2C_REAL, nullptr, c->V, c->aux_vt, c->rows, 1.0, 0.0, c->stream);
+    else
+        c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->aux_vt, c->rows, 1.0, 0.0, c->stream);
+    vt_forward(c, c->aux_vt);
+    HIPCHK(hipMemsetAsync(c->spec_P, 0, sizeof(double) * nbins, c->stream));
+    HIPCHK(hipMemsetAsync(c->spec_cnt, 0, sizeof(unsigned long long) * nbins, c->stream));
+    const double norm = 1.0 / ((double)c->S * (double)c->S);
+    const size_t shm = (size_t)nbins * (sizeof(double) + sizeof(unsigned long long));
+    const unsigned grid = grid_for(c->H, 256, 2048);
+    if (fp64(c))
+        spectrum_kernel<double><<<grid, 256, shm, c->stream>>>((const double2 *)c->aux_vt, c->rows, c->N, c->dim, nbins, spb, norm, c->spec_P, c->spec_cnt);
+    else
+        spectrum_kernel<float><<<grid, 256, shm, c->stream>>>((const float2 *)c->aux_vt, c->rows, c->N, c->dim, nbins, spb, norm, c->spec_P, c->spec_cnt);
+    std::vector<unsigned long long> cnt(nbins);
+    HIPCHK(hipMemcpyAsync(Pk, c->spec_P, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->spec_cnt, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    const double dk = TWO_PI / c->L;
+    for (int b = 0; b < nbins; ++b) {
+        count[b] = (int64_t)cnt[b];
+        if (cnt[b] > 0) Pk[b] /= (double)cnt[b];
+        kmodes[b] = ((double)b * spb + 0.5 * (spb - 1)) * dk;
+    }
+    return QX_OK;
+}
+
+// ---- step compositions --------------------------------------------------------
+
+qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)
+{
+    CHECK_CTX(c);
+    if (nsteps < 0) return fail(QX_EINVAL, "nsteps must be >= 0");
+    const double k_first = first_half ? 0.5 * ds : ds;
+    if (nsteps == 0) {
+        TRY(qx_kick(c, k_first));
+        if (last_half) TRY(qx_kick(c, 0.5 * ds));
+        return QX_OK;
+    }
+    const MulArgs pm = poisson_args(c);
+    const MulArgs dm = drift_args(c, ds);
+    MulArgs none{};
+    const int N = c->N;
+    for (int i = 0; i < nsteps; ++i) {
+        // K (from V on entry, from the potential's spectrum afterwards) + forward last axis
+        if (i == 0)
+            c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, k_first, C2R_KICK | C2R_FWD, nullptr, c->stream);
+        else
+            c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, ds, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream);
+        // D on the leading axes
+        if (c->dim == 3) {
+            c->eng->strided(c->psi, N, N, -1, none, c->stream);
+            c->eng->strided(c->psi, 1, (long long)N * N, 0, dm, c->stream);
+            c->eng->strided(c->psi, N, N, +1, none, c->stream);
+        } else {
+            c->eng->strided(c->psi, 1, N, 0, dm, c->stream);
+        }
+        // end of D (inverse last axis) fused with the density and its r2c
+        c->eng->r2c_rows(R2C_INV_DENSITY, c->psi, nullptr, c->vt, c->rows, c->ph.pref, rho_bar_eff(c), c->stream);
+        // Poisson on the half spectrum
+        if (c->dim == 3) {
+            c->eng->contig(c->vt, c->H1 * N, -1, none, c->stream);
+            c->eng->strided(c->vt, c->H1, N, 0, pm, c->stream);
+            c->eng->contig(c->vt, c->H1 * N, +1, none, c->stream);
+        } else {
+            c->eng->contig(c->vt, c->H1, 0, pm, c->stream);
+        }
+    }
+    TRY(reset_vmax(c));
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, 0.5 * ds,
+                     C2R_SRC_VT | C2R_STORE_V | C2R_VMAX | (last_half ? C2R_KICK : 0), c->vmax_bits,
+                     c->stream);
+    return read_vmax(c);
+}
+
+qx_status qx_propagate(qx_ctx *c, double ds)
+{
+    CHECK_CTX(c);
+    TRY(qx_kick(c, 0.5 * ds));
+    TRY(qx_drift(c, ds));
+    TRY(potential_from_psi(c));
+    return qx_kick(c, 0.5 * ds);
+}
+
+// ---- FFTBackend -------------------------------------------------------------------
+
+qx_status qx_fft_c2c(qx_ctx *c, double *data, int sign)
+{
+    CHECK_CTX(c);
+    if (!data) return fail(QX_EINVAL, "null data");
+    if (sign != -1 && sign != 1) return fail(QX_EINVAL, "sign must be -1 or +1");
+    TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+    TRY(upload_c(c, data, c->aux_c, c->S));
+    c2c_passes(c, c->aux_c, sign);
+    TRY(sync(c));
+    return download_c(c, c->aux_c, data, c->S);
+}
+
+static qx_status vt_to_ref_launch(qx_ctx *c, const void *vt, void *ref)
+{
+    dim3 blk(32, 8), grd((unsigned)((c->rows + 31) / 32), (unsigned)((c->H1 + 31) / 32));
+    if (fp64(c)) vt_to_ref<double><<<grd, blk, 0, c->stream>>>((const double2 *)vt, (double2 *)ref, c->rows, (int)c->H1);
+    else vt_to_ref<float><<<grd, blk, 0, c->stream>>>((const float2 *)vt, (float2 *)ref, c->rows, (int)c->H1);
+    return QX_OK;
+}
+static qx_status ref_to_vt_launch(qx_ctx *c, const void *ref, void *vt)
+{
+    dim3 blk(32, 8), grd((unsigned)((c->rows + 31) / 32), (unsigned)((c->H1 + 31) / 32));
+    if (fp64(c)) ref_to_vt<double><<<grd, blk, 0, c->stream>>>((const double2 *)ref, (double2 *)vt, c->rows, (int)c->H1);
+    else ref_to_vt<float><<<grd, blk, 0, c->stream>>>((const float2 *)ref, (float2 *)vt, c->rows, (int)c->H1);
+    return QX_OK;
+}
+
+qx_status qx_fft_r2c(qx_ctx *c, const double *in, double *out)
+{
+    CHECK_CTX(c);
+    if (!in || !out) return fail(QX_EINVAL, "null argument");
+    TRY(ensure(c, &c->aux_r, (size_t)c->S * c->rsz));
+    TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
+    TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+    TRY(upload_r(c, in, c->aux_r, c->S));
+    c->eng->r2c_rows(R2C_REAL, nullptr, c->aux_r, c->aux_vt, c->rows, 1.0, 0.0, c->stream);
+    vt_forward(c, c->aux_vt);
+    TRY(vt_to_ref_launch(c, c->aux_vt, c->aux_c));
+    TRY(sync(c));
+    return download_c(c, c->aux_c, out, c->H);
+}
+
+qx_status qx_fft_c2r(qx_ctx *c, const double *in, double *out)
+{
+    CHECK_CTX(c);
+    if (!in || !out) return fail(QX_EINVAL, "null argument");
+    TRY(ensure(c, &c->aux_r, (size_t)c->S * c->rsz));
+    TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
+    TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+    TRY(upload_c(c, in, c->aux_c, c->H));
+    TRY(ref_to_vt_launch(c, c->aux_c, c->aux_vt));
+    vt_inverse(c, c->aux_vt);
+    c->eng->c2r_rows(c->aux_vt, c->aux_r, nullptr, c->rows, 0.0, C2R_SRC_VT | C2R_STORE_V, nullptr, c->stream);
+    TRY(sync(c));
+    return download_r(c, c->aux_r, out, c->S);
+}
+
+int64_t qx_sites(const qx_ctx *c) { return c ? c->S : 0; }
+int64_t qx_ksites(const qx_ctx *c) { return c ? c->H : 0; }
+int qx_global_N(const qx_ctx *c) { return c ? c->N : 0; }
+
+// ---- initial conditions -------------------------------------------------------------
+
+qx_status qx_ic_solitons(qx_ctx *c, const double *sol, int nsol, double bkg)
+{
+    CHECK_CTX(c);
+    if (nsol < 0 || (nsol > 0 && !sol)) return fail(QX_EINVAL, "bad soliton table");
+    TRY(ensure(c, &c->stage, (size_t)STAGE_ELEMS * 16));
+    if ((size_t)nsol * 5 * sizeof(double) > (size_t)STAGE_ELEMS * 16) return fail(QX_EINVAL, "too many solitons");
+    if (nsol) HIPCHK(hipMemcpyAsync(c->stage, sol, sizeof(double) * 5 * nsol, hipMemcpyHostToDevice, c->stream));
+    const unsigned g = grid_for(c->S);
+    if (fp64(c))
+        solitons_rho_kernel<double><<<g, 256, 0, c->stream>>>((double *)c->V, (const double *)c->stage, nsol, c->N, c->dim, c->L, bkg, c->S);
+    else
+        solitons_rho_kernel<float><<<g, 256, 0, c->stream>>>((float *)c->V, (const double *)c->stage, nsol, c->N, c->dim, c->L, bkg, c->S);
+    double sum = 0.0;
+    TRY(reduce<RED_VABS>(c, nullptr, c->V, c->S, &sum, nullptr));
+    const double inv_mean = (double)c->S / sum;
+    if (fp64(c))
+        solitons_psi_kernel<double><<<g, 256, 0, c->stream>>>((double2 *)c->psi, (const double *)c->V, inv_mean, c->S);
+    else
+        solitons_psi_kernel<float><<<g, 256, 0, c->stream>>>((float2 *)c->psi, (const float *)c->V, inv_mean, c->S);
+    HIPCHK(hipMemsetAsync(c->V, 0, (size_t)c->S * c->rsz, c->stream));
+    return sync(c);
+}
+
+qx_status qx_ic_grf(qx_ctx *c, uint64_t seed, double amp)
+{
+    CHECK_CTX(c);
+    TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+    const unsigned g = grid_for(c->S);
+    if (fp64(c))
+        grf_noise_kernel<double><<<g, 256, 0, c->stream>>>((double2 *)c->psi, (double2 *)c->aux_c, seed, c->S);
+    else
+        grf_noise_kernel<float><<<g, 256, 0, c->stream>>>((float2 *)c->psi, (float2 *)c->aux_c, seed, c->S);
+    c2c_passes(c, c->psi, -1);
+    c2c_passes(c, c->aux_c, -1);
+    if (fp64(c)) {
+        radial_filter_kernel<double><<<g, 256, 0, c->stream>>>((double2 *)c->psi, c->N, c->dim, 0, c->S);
+        radial_filter_kernel<double><<<g, 256, 0, c->stream>>>((double2 *)c->aux_c, c->N, c->dim, 1, c->S);
+    } else {
+        radial_filter_kernel<float><<<g, 256, 0, c->stream>>>((float2 *)c->psi, c->N, c->dim, 0, c->S);
+        radial_filter_kernel<float><<<g, 256, 0, c->stream>>>((float2 *)c->aux_c, c->N, c->dim, 1, c->S);
+    }
+    c2c_passes(c, c->psi, +1);
+    c2c_passes(c, c->aux_c, +1);
+    const double inv_s = 1.0 / (double)c->S;
+    const unsigned blocks = grid_for(c->S, RED_THREADS, MAX_PART);
+    if (fp64(c))
+        moments_kernel<double><<<blocks, RED_THREADS, 0, c->stream>>>((const double2 *)c->psi, inv_s, c->S, c->partials);
+    else
+        moments_kernel<float><<<blocks, RED_THREADS, 0, c->stream>>>((const float2 *)c->psi, inv_s, c->S, c->partials);
+    HIPCHK(hipMemcpyAsync(c->h_partials, c->partials, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    double s1 = 0.0, s2 = 0.0;
+    for (unsigned b = 0; b < blocks; ++b) { s1 += c->h_partials[2 * b]; s2 += c->h_partials[2 * b + 1]; }
+    const double mean = s1 * inv_s, var = s2 * inv_s - mean * mean;
+    if (!(var > 0.0)) return fail(QX_EINVAL, "degenerate random field (zero variance)");
+    if (fp64(c))
+        grf_compose_kernel<double><<<g, 256, 0, c->stream>>>((double2 *)c->psi, (const double2 *)c->aux_c, inv_s, mean, 1.0 / std::sqrt(var), amp, c->S);
+    else
+        grf_compose_kernel<float><<<g, 256, 0, c->stream>>>((float2 *)c->psi, (const float2 *)c->aux_c, inv_s, mean, 1.0 / std::sqrt(var), amp, c->S);
+    return sync(c);
+}
+
+qx_status qx_ic_blobs(qx_ctx *c, const double *tab, int nb, double bkg)
+{
+    CHECK_CTX(c);
+    if (nb < 0 || (nb > 0 && !tab)) return fail(QX_EINVAL, "bad blob table");
+    TRY(ensure(c, &c->stage, (size_t)STAGE_ELEMS * 16));
+    if ((size_t)nb * 6 * sizeof(double) > (size_t)STAGE_ELEMS * 16) return fail(QX_EINVAL, "too many blobs");
+    if (nb) HIPCHK(hipMemcpyAsync(c->stage, tab, sizeof(double) * 6 * nb, hipMemcpyHostToDevice, c->stream));
+    const unsigned g = grid_for(c->S);
+    if (fp64(c))
+        blobs_kernel<double><<<g, 256, 0, c->stream>>>((double2 *)c->psi, (const double *)c->stage, nb, c->N, c->dim, c->L, bkg, c->S);
+    else
+        blobs_kernel<float><<<g, 256, 0, c->stream>>>((float2 *)c->psi, (const double *)c->stage, nb, c->N, c->dim, c->L, bkg, c->S);
+    return sync(c);
+}
+
+// ---- plumbing -----------------------------------------------------------------------
+
+void *qx_stream(qx_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+qx_status qx_synchronize(qx_ctx *c)
+{
+    CHECK_CTX(c);
+    return sync(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// fieldtest.cxx -- the reference's pybind11 test module
+// (src/examples/fieldtest.cxx:11-105): each entry point builds a Field from
+// defaults() with the soliton IC, runs one operator sequence and returns one
+// real scalar.  `fp32` (keyword, default False) selects complex64 device state.
+#include <pybind11/pybind11.h>
+#include "field.h"
+#include "ic.h"
+
+namespace py = pybind11;
+
+namespace {
+std::unique_ptr<Field> make_field(int n, int N, bool fp32)
+{
+    Params pars;
+    defaults(pars);
+    pars.N = N;
+    pars.dim = n;
+    pars.norm = 1;
+    pars.verb = false;
+    pars.precision = fp32 ? Precision::FP32 : Precision::FP64;
+    auto field = std::make_unique<Field>(pars);
+    auto ic = createIC(pars.ictype, pars);
+    ic->apply(*field);
+    return field;
+}
+}  // namespace
+
+double test_kick(int n, int N, double dt, int idx, bool fp32)
+{
+    auto field = make_field(n, N, fp32);
+    field->updatePotential();
+    field->kick(dt);
+    return field->getPsiIdx(idx);
+}
+
+double test_drift(int n, int N, double dt, int idx, bool fp32)
+{
+    auto field = make_field(n, N, fp32);
+    field->drift(dt);
+    return field->getPsiIdx(idx);
+}
+
+double test_potential(int n, int N, int idx, bool fp32)
+{
+    auto field = make_field(n, N, fp32);
+    field->updatePotential();
+    return field->getVIdx(idx);
+}
+
+double test_kick_drift_step(int n, int N, double dt, int idx, bool fp32)
+{
+    auto field = make_field(n, N, fp32);
+    field->updatePotential();
+    field->kick(0.5 * dt);
+    field->drift(dt);
+    field->updatePotential();
+    field->kick(0.5 * dt);
+    return field->getPsiIdx(idx);
+}
+
+PYBIND11_MODULE(fieldtest, m)
+{
+    m.def("test_kick", &test_kick, "", py::arg("n"), py::arg("N"), py::arg("dt"), py::arg("idx"),
+          py::arg("fp32") = false);
+    m.def("test_drift", &test_drift, "", py::arg("n"), py::arg("N"), py::arg("dt"), py::arg("idx"),
+          py::arg("fp32") = false);
+    m.def("test_potential", &test_potential, "", py::arg("n"), py::arg("N"), py::arg("idx"),
+          py::arg("fp32") = false);
+    m.def("test_kick_drift_step", &test_kick_drift_step, "", py::arg("n"), py::arg("N"),
+          py::arg("dt"), py::arg("idx"), py::arg("fp32") = false);
+}
