@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the fused kick-drift-kick Schroedinger-Poisson step on one MI355X.
+
+Workload (default): 3D lattice 512^3, L = 40, complex64 state with FP64
+phase reduction and sums, dt = 2.5e-4, seeded Gaussian-random-field IC
+(P(k) ~ k^-2 cut at k_Ny/2), code units pref = 1, rho_bar = 1, k = 0 mode
+zeroed.  A step is one merged-kick KDK step of the reference time loop
+(src/main.cxx:65-85): K, D (forward FFT, e^{-i dt k^2/2}, inverse FFT),
+P (|psi|^2, r2c, -1/k^2, c2r, max|V|).
+
+Prints ONE JSON line: site-updates/s (value), ms per step, the roofline of
+the dominant pass and of the whole step, and the FP64 CPU oracle timed on
+the host cores on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def algorithmic_bytes_per_site(N, c=8, r=4):
+    """Each operator of one merged-kick step reads its inputs and writes its
+    outputs once: B = (9c + 5r) S + 4c H, H = S (N/2+1)/N."""
+    return (9 * c + 5 * r) + 4 * c * (N // 2 + 1) / N
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (the library's stream)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        for n in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize",
+                  "hipEventElapsedTime", "hipEventDestroy"):
+            getattr(self.h, n).restype = ctypes.c_int
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.h.hipEventSynchronize(e1) == 0
+        t = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(t), e0, e1) == 0
+        return t.value
+
+
+def cpu_baseline(lat, N, L, dt, steps, threads):
+    """FP64 oracle (oracle/qx_oracle.c, OpenMP) on the host: `steps` merged-kick
+    steps of the same lattice and IC, timed with a wall clock."""
+    import numpy as np
+    import oracle
+    oracle.set_threads(threads)
+    psi = lat.psi
+    V = lat.V
+    t0 = time.perf_counter()
+    oracle.leapfrog(psi, V, L, dt, steps, 1.0, 1.0, 0, 0)
+    t = time.perf_counter() - t0
+    del psi, V
+    return N ** 3 * steps / t, t, np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--N", type=int, default=512)
+    ap.add_argument("--dim", type=int, default=3)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pass-reps", type=int, default=10)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
+              "multi-GPU slab decomposition is a design only (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import qaxions_amd as qx
+
+    N, dim, L, dt = args.N, args.dim, 40.0, 2.5e-4
+    S = N ** dim
+    print(f"[bench] {dim}D {N}^{dim} {args.precision} L={L} dt={dt}", file=sys.stderr, flush=True)
+    lat = qx.Lattice(dim, N, L, precision=args.precision, pref=1.0, rho_bar=1.0, k0_mode=0)
+    lat.ic_grf(seed=0, amp=0.1)
+    lat.update_potential()
+    r0, m0 = lat.density_stats()
+    ev = HipEvents()
+    stream = lat.stream
+
+    # warmup: a segment left open (continues with K(ds))
+    lat.leapfrog(dt, max(args.warmup, 1), first_half=True, last_half=False)
+    lat.synchronize()
+    e0, e1 = ev.event(), ev.event()
+    w0 = time.perf_counter()
+    ev.record(e0, stream)
+    lat.leapfrog(dt, args.steps, first_half=False, last_half=True)
+    ev.record(e1, stream)
+    ms_total = ev.elapsed_ms(e0, e1)
+    lat.synchronize()
+    wall = time.perf_counter() - w0
+    ms_step = ms_total / args.steps
+    value = S / (ms_step * 1e-3)
+    r1, m1 = lat.density_stats()
+    print(f"[bench] {ms_step:.4f} ms/step (events), wall {wall*1e3/args.steps:.4f} ms/step, "
+          f"mass drift {abs(m1-m0)/m0:.2e}", file=sys.stderr, flush=True)
+
+    # per-pass live timing: the dominant pass and its roofline
+    names, pms, pbytes = lat.time_step_passes(dt, reps=args.pass_reps)
+    dom = max(range(len(names)), key=lambda i: pms[i])
+    for n_, t_, b_ in zip(names, pms, pbytes):
+        print(f"[bench] pass {n_:>18s}: {t_*1e3:8.1f} us  {b_/t_/1e9:8.1f} GB/s",
+              file=sys.stderr, flush=True)
+    csz = 8 if args.precision == "fp32" else 16
+    B_site = algorithmic_bytes_per_site(N, c=csz, r=csz // 2)
+    step_achieved = B_site * S / (ms_step * 1e-3)
+
+    out = {
+        "metric": "site-updates/s per KDK step at 512^3 (FP32 state, FP64 phase reduction and sums)",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" if args.precision == "fp32" else "c128",
+        "data": "synthetic: seeded Gaussian random field IC, P(k)~k^-2 cut at k_Ny/2",
+        "config": {"workload": f"KDK Schroedinger-Poisson step, {dim}D {N}^{dim}, L={L}, dt={dt}, "
+                               f"pref=1, rho_bar=1, k=0 zeroed, GRF seed 0",
+                   "lattice": [N] * dim, "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": names[dom],
+                     "achieved": pbytes[dom] / (pms[dom] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": pbytes[dom] / (pms[dom] * 1e-3) / HBM_PEAK,
+                     "traffic": None,
+                     "bytes_per_launch": pbytes[dom], "ms_per_launch": pms[dom]},
+        "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_site": B_site,
+                          "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "frac": step_achieved / HBM_PEAK},
+        "passes": {n_: {"us": t_ * 1e3, "GBps": b_ / t_ / 1e6} for n_, t_, b_ in zip(names, pms, pbytes)},
+        "mass_drift": abs(m1 - m0) / m0,
+        "wall_ms_per_step": wall * 1e3 / args.steps,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline: {args.cpu_steps} steps of the FP64 oracle on {threads} threads",
+              file=sys.stderr, flush=True)
+        v, t, _ = cpu_baseline(lat, N, L, dt, args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "site-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_steps} merged-kick KDK steps of the FP64 C oracle "
+                                         f"(radix-2 FFT, OpenMP) on the same {N}^{dim} lattice and IC, "
+                                         f"{t:.1f} s wall"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -132,6 +132,17 @@ qx_status qx_ic_grf(qx_ctx *ctx, uint64_t seed, double amp);
 /* blobs: rows (rho_c, r_c, x, y, z, phi), background density bkg */
 qx_status qx_ic_blobs(qx_ctx *ctx, const double *table, int nblobs, double bkg);
 
+/* ---- measurement (no reference counterpart; replaces the reference's
+ * PROFILE/FLOP_COUNT section timers, src/profiler.h:158-185) -------------- */
+/* passes of one fused steady-state KDK step (8 in 3D, 4 in 2D) */
+int qx_step_pass_count(const qx_ctx *ctx);
+const char *qx_step_pass_name(const qx_ctx *ctx, int pass);
+/* Times each pass alone, `reps` launches back to back on scratch copies of
+ * the state (the state itself is left untouched), with HIP events on the
+ * context's stream.  ms[pass] = average ms per launch; bytes[pass]
+ * (nullable) = minimum HBM bytes the pass moves.  cap >= pass count. */
+qx_status qx_time_step_passes(qx_ctx *ctx, double ds, int reps, double *ms, double *bytes, int cap);
+
 /* ---- plumbing (no reference counterpart) -------------------------------- */
 void *qx_stream(qx_ctx *ctx);          /* the context's hipStream_t */
 qx_status qx_synchronize(qx_ctx *ctx);

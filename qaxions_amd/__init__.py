@@ -77,6 +77,9 @@ SIGNATURES = {
     "qx_ic_solitons": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double]),
     "qx_ic_grf": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double]),
     "qx_ic_blobs": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double]),
+    "qx_step_pass_count": (C.c_int, [C.c_void_p]),
+    "qx_step_pass_name": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "qx_time_step_passes": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp, _dp, C.c_int]),
     "qx_stream": (C.c_void_p, [C.c_void_p]),
     "qx_synchronize": (C.c_int, [C.c_void_p]),
     "qx_last_error": (C.c_char_p, []),
@@ -288,6 +291,16 @@ class Lattice:
     def ic_blobs(self, table, bkg=0.1):
         t = np.ascontiguousarray(np.asarray(table, np.float64).reshape(-1, 6))
         _check(lib().qx_ic_blobs(self._h, _p(t), t.shape[0], float(bkg)))
+
+    # -- measurement ------------------------------------------------------------------------
+    def time_step_passes(self, ds, reps=10):
+        """Per-pass average ms (HIP events) and minimum bytes of the fused step."""
+        n = int(lib().qx_step_pass_count(self._h))
+        ms = np.zeros(n)
+        by = np.zeros(n)
+        _check(lib().qx_time_step_passes(self._h, float(ds), int(reps), _p(ms), _p(by), n))
+        names = [lib().qx_step_pass_name(self._h, i).decode() for i in range(n)]
+        return names, ms, by
 
     # -- plumbing ---------------------------------------------------------------------------
     @property

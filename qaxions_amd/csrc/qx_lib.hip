@@ -317,6 +317,57 @@ qx_status reduce(qx_ctx *c, const void *psi, const void *V, long long n, double 
 
 bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
 
+// ---- the fused steady-state step, pass by pass ---------------------------
+int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 8 : 4; }
+
+// pass p of one fused KDK step acting on (psi, vt); see the file header
+void step_pass(qx_ctx *c, int p, double ds, void *psi, void *vt, const MulArgs &dm, const MulArgs &pm)
+{
+    MulArgs none{};
+    const int N = c->N;
+    const long long NN = (long long)N * N;
+    if (c->dim == 3) {
+        switch (p) {
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, ds, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
+        case 1: c->eng->strided(psi, N, N, -1, none, c->stream); break;
+        case 2: c->eng->strided(psi, 1, NN, 0, dm, c->stream); break;
+        case 3: c->eng->strided(psi, N, N, +1, none, c->stream); break;
+        case 4: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 5: c->eng->contig(vt, c->H1 * N, -1, none, c->stream); break;
+        case 6: c->eng->strided(vt, c->H1, N, 0, pm, c->stream); break;
+        case 7: c->eng->contig(vt, c->H1 * N, +1, none, c->stream); break;
+        }
+    } else {
+        switch (p) {
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, ds, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
+        case 1: c->eng->strided(psi, 1, N, 0, dm, c->stream); break;
+        case 2: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
+        }
+    }
+}
+
+// bytes each pass must move at minimum (every input read once, every output
+// written once): psi S complex, vt H complex
+double pass_bytes(const qx_ctx *c, int p)
+{
+    const double S = (double)c->S, H = (double)c->H, cs = (double)c->csz;
+    if (c->dim == 3) {
+        switch (p) {
+        case 0: return H * cs + 2 * S * cs;   // read vt, rw psi
+        case 4: return 2 * S * cs + H * cs;   // rw psi, write vt
+        case 1: case 2: case 3: return 2 * S * cs;
+        default: return 2 * H * cs;
+        }
+    }
+    switch (p) {
+    case 0: return H * cs + 2 * S * cs;
+    case 1: return 2 * S * cs;
+    case 2: return 2 * S * cs + H * cs;
+    default: return 2 * H * cs;
+    }
+}
+
 #define CHECK_CTX(c)                                                  \
     do {                                                              \
         if (!valid_ctx(c)) return fail(QX_EINVAL, "null context");    \
@@ -627,38 +678,64 @@ qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last
     }
     const MulArgs pm = poisson_args(c);
     const MulArgs dm = drift_args(c, ds);
-    MulArgs none{};
-    const int N = c->N;
+    const int np = step_pass_count(c);
     for (int i = 0; i < nsteps; ++i) {
-        // K (from V on entry, from the potential's spectrum afterwards) + forward last axis
+        // the opening kick reads the stored V; later kicks the potential's spectrum
         if (i == 0)
             c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, k_first, C2R_KICK | C2R_FWD, nullptr, c->stream);
         else
-            c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, ds, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream);
-        // D on the leading axes
-        if (c->dim == 3) {
-            c->eng->strided(c->psi, N, N, -1, none, c->stream);
-            c->eng->strided(c->psi, 1, (long long)N * N, 0, dm, c->stream);
-            c->eng->strided(c->psi, N, N, +1, none, c->stream);
-        } else {
-            c->eng->strided(c->psi, 1, N, 0, dm, c->stream);
-        }
-        // end of D (inverse last axis) fused with the density and its r2c
-        c->eng->r2c_rows(R2C_INV_DENSITY, c->psi, nullptr, c->vt, c->rows, c->ph.pref, rho_bar_eff(c), c->stream);
-        // Poisson on the half spectrum
-        if (c->dim == 3) {
-            c->eng->contig(c->vt, c->H1 * N, -1, none, c->stream);
-            c->eng->strided(c->vt, c->H1, N, 0, pm, c->stream);
-            c->eng->contig(c->vt, c->H1 * N, +1, none, c->stream);
-        } else {
-            c->eng->contig(c->vt, c->H1, 0, pm, c->stream);
-        }
+            step_pass(c, 0, ds, c->psi, c->vt, dm, pm);
+        for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
     }
+    // last step's c2r: materialise V and max|V|, closing half kick
     TRY(reset_vmax(c));
     c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, 0.5 * ds,
                      C2R_SRC_VT | C2R_STORE_V | C2R_VMAX | (last_half ? C2R_KICK : 0), c->vmax_bits,
                      c->stream);
     return read_vmax(c);
+}
+
+int qx_step_pass_count(const qx_ctx *c) { return valid_ctx(c) ? step_pass_count(c) : 0; }
+
+const char *qx_step_pass_name(const qx_ctx *c, int p)
+{
+    static const char *n3[] = {"c2r_kick_fwd2", "fwd1", "drift0", "inv1", "inv2_density_r2c",
+                               "vt_fwd1", "vt_poisson0", "vt_inv1"};
+    static const char *n2[] = {"c2r_kick_fwd1", "drift0", "inv1_density_r2c", "vt_poisson0"};
+    if (!valid_ctx(c) || p < 0 || p >= step_pass_count(c)) return "";
+    return c->dim == 3 ? n3[p] : n2[p];
+}
+
+qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double *bytes, int cap)
+{
+    CHECK_CTX(c);
+    const int np = step_pass_count(c);
+    if (reps < 1 || !ms || cap < np) return fail(QX_EINVAL, "need reps >= 1 and room for every pass");
+    TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+    TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
+    HIPCHK(hipMemcpyAsync(c->aux_c, c->psi, (size_t)c->S * c->csz, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->aux_vt, c->vt, (size_t)c->H * c->csz, hipMemcpyDeviceToDevice, c->stream));
+    const MulArgs pm = poisson_args(c);
+    const MulArgs dm = drift_args(c, ds);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    qx_status st = QX_OK;
+    for (int p = 0; p < np && st == QX_OK; ++p) {
+        step_pass(c, p, ds, c->aux_c, c->aux_vt, dm, pm);  // warm
+        if (hipEventRecord(e0, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
+        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->aux_vt, dm, pm);
+        if (hipEventRecord(e1, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
+        float t = 0.f;
+        if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            st = fail(QX_EDEVICE, "event timing failed");
+        ms[p] = (double)t / reps;
+        if (bytes) bytes[p] = pass_bytes(c, p);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != QX_OK) return st;
+    return sync(c);
 }
 
 qx_status qx_propagate(qx_ctx *c, double ds)

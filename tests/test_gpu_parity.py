@@ -1,0 +1,282 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle on the same
+seeded inputs.  Tolerances: FP64 contexts 1e-12 relative, FP32 contexts as
+stated per test (<= 1e-5 relative L2 after 100 steps)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as o
+import refmath as rm
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(7)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def lattice(qx, dim, N, L=1.0, prec="fp64", **ph):
+    print(f"[case] dim={dim} N={N} L={L} prec={prec}", flush=True)
+    return qx.Lattice(dim, N, L, precision=prec, **ph)
+
+
+TOL = {"fp64": 1e-12, "fp32": 2e-6}
+
+FFT_CASES = [(2, n) for n in (2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096)] + \
+            [(3, n) for n in (2, 4, 8, 16, 32, 64, 128)]
+
+
+# ------------------------------------------------------------------ transforms
+@pytest.mark.parametrize("dim,N", FFT_CASES)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fft_c2c(qx, dim, N, prec):
+    lat = lattice(qx, dim, N, prec=prec)
+    x = rng.standard_normal((N,) * dim) + 1j * rng.standard_normal((N,) * dim)
+    for sign in (-1, 1):
+        want = o.fft_c2c(x, sign)
+        got = lat.fft_c2c(x, sign)
+        assert rel_l2(got, want) < TOL[prec] * max(1.0, math.log2(N)), (sign, rel_l2(got, want))
+
+
+@pytest.mark.parametrize("dim,N", FFT_CASES)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fft_r2c_c2r(qx, dim, N, prec):
+    lat = lattice(qx, dim, N, prec=prec)
+    x = rng.standard_normal((N,) * dim)
+    H = lat.fft_r2c(x)
+    Hw = o.fft_r2c(x)
+    assert rel_l2(H, Hw) < TOL[prec] * max(1.0, math.log2(N))
+    back = lat.fft_c2r(Hw)
+    assert rel_l2(back, o.fft_c2r(Hw)) < TOL[prec] * max(1.0, math.log2(N))
+
+
+# ------------------------------------------------------------------ operators
+OP_CASES = [(2, 8, 1.0), (2, 64, 10.0), (2, 256, 10.0), (3, 8, 1.0), (3, 32, 20.0)]
+
+
+@pytest.mark.parametrize("dim,N,L", OP_CASES)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_kick_drift_potential(qx, dim, N, L, prec):
+    pref = 4 * math.pi if L == 1.0 else 1.0
+    lat = lattice(qx, dim, N, L, prec, pref=pref)
+    psi = o.ic_grf(dim, N, L, 3) if N >= 16 else o.ic_solitons(dim, N, L, rm.read_solitons())
+    lat.psi = psi
+    vmax = lat.update_potential()
+    V, _, vmax_o = o.update_potential(psi, L, pref)
+    assert rel_l2(lat.V, V) < 10 * TOL[prec]
+    assert vmax == pytest.approx(vmax_o, rel=10 * TOL[prec])
+    lat.kick(0.1)
+    p1 = o.kick(psi, V, 0.1)
+    assert rel_l2(lat.psi, p1) < TOL[prec]
+    lat.drift(0.05)
+    p2 = o.drift(p1, L, 0.05)
+    assert rel_l2(lat.psi, p2) < 2 * TOL[prec] * math.log2(N)
+
+
+@pytest.mark.parametrize("k0_mode,rho_bar_mode", [(1, 0), (0, 0), (1, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_potential_conventions(qx, k0_mode, rho_bar_mode, prec):
+    dim, N, L = 3, 16, 20.0
+    tab = o.blob_table(4, 1, L)
+    psi = o.ic_blobs(dim, N, L, tab, 0.1)  # <rho> != 1
+    lat = lattice(qx, dim, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=k0_mode,
+                  rho_bar_mode=rho_bar_mode)
+    lat.psi = psi
+    lat.update_potential()
+    V, _, _ = o.update_potential(psi, L, 1.0, 1.0, k0_mode, rho_bar_mode)
+    assert rel_l2(lat.V, V) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_poisson_half_and_compute_energy(qx, prec):
+    dim, N, L = 2, 32, 10.0
+    lat = lattice(qx, dim, N, L, prec, pref=1.0)
+    vh = o.fft_r2c(rng.standard_normal((N, N)))
+    got = lat.poisson_half(vh)
+    assert rel_l2(got, o.poisson(vh, L, 1)) < TOL[prec]
+    psi = o.ic_grf(dim, N, L, 9)
+    lat.psi = psi
+    r = lat.compute_energy()
+    Vo, ro = o.compute_energy(psi)
+    assert r == pytest.approx(ro, rel=10 * TOL[prec])
+    assert rel_l2(lat.V, Vo) < TOL[prec]
+    rr, mass = lat.density_stats()
+    ro2, s = o.density_stats(psi)
+    assert rr == pytest.approx(ro2, rel=10 * TOL[prec])
+    assert mass == pytest.approx(s * (L / N) ** 2, rel=10 * TOL[prec])
+
+
+@pytest.mark.parametrize("dim,N,L", [(2, 64, 10.0), (3, 16, 20.0)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_energy(qx, dim, N, L, prec):
+    lat = lattice(qx, dim, N, L, prec, pref=1.0)
+    psi = o.ic_grf(dim, N, L, 21)
+    lat.psi = psi
+    lat.update_potential()
+    V, _, _ = o.update_potential(psi, L, 1.0)
+    m, k, p = lat.energy()
+    mo, ko, po = o.mass_energy(psi, V, L)
+    assert m == pytest.approx(mo, rel=10 * TOL[prec])
+    assert k == pytest.approx(ko, rel=10 * TOL[prec])
+    assert p == pytest.approx(po, rel=1e3 * TOL[prec], abs=1e-9)
+
+
+@pytest.mark.parametrize("dim,N,nbins,spb", [(2, 16, 9, 1), (2, 256, 64, 2), (3, 16, 9, 1),
+                                             (3, 32, 8, 2), (2, 10, 6, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_spectrum(qx, dim, N, nbins, spb, prec):
+    if N & (N - 1):
+        with pytest.raises(qx.QxError):
+            qx.Lattice(dim, N, 1.0, precision=prec)
+        return
+    L = 10.0
+    lat = lattice(qx, dim, N, L, prec)
+    psi = o.ic_grf(dim, N, L, 5)
+    lat.psi = psi
+    P, c, k = lat.spectrum(source="rho", nbins=nbins, spb=spb)
+    Po, co, ko = o.spectrum(np.abs(psi) ** 2, L, nbins, spb)
+    assert np.array_equal(c, co)
+    assert np.allclose(k, ko, rtol=1e-15)
+    nz = co > 0
+    assert np.abs(P[nz] / Po[nz] - 1).max() < (1e-11 if prec == "fp64" else 2e-3)
+    # the reference's choice: spectrum of V as it stands after computeEnergy
+    lat.compute_energy()
+    P2, c2, _ = lat.spectrum(source="V", nbins=nbins, spb=spb)
+    assert np.array_equal(c2, co)
+    assert np.abs(P2[nz] / Po[nz] - 1).max() < (1e-11 if prec == "fp64" else 2e-3)
+
+
+# ------------------------------------------------------------------ reference tests
+@pytest.mark.parametrize("n,N,dt,idx", rm.PARAMS)
+@pytest.mark.parametrize("fp32", [False, True])
+def test_reference_fieldtest_module(n, N, dt, idx, fp32, in_golden_dir):
+    """tests/test_field.py:12-129 through the pybind11 module, same tolerance."""
+    import qaxions_amd.fieldtest as field
+    g = np.load("golden_field.npz")
+    assert field.test_kick(n, N, dt, idx, fp32=fp32) == pytest.approx(
+        g[f"kick_{n}_{N}_{dt}_{idx}"], abs=1e-5, rel=1e-4)
+    assert field.test_drift(n, N, dt, idx, fp32=fp32) == pytest.approx(
+        g[f"drift_{n}_{N}_{dt}_{idx}"], abs=1e-5, rel=1e-4)
+    assert field.test_kick_drift_step(n, N, dt, idx, fp32=fp32) == pytest.approx(
+        g[f"kds_{n}_{N}_{dt}_{idx}"], abs=1e-5, rel=1e-4)
+    if not fp32:  # FP64 device state: parity at the oracle's level
+        assert field.test_kick_drift_step(n, N, dt, idx) == pytest.approx(
+            g[f"kds_{n}_{N}_{dt}_{idx}"], abs=1e-12)
+
+
+@pytest.mark.parametrize("n,N,idx", rm.PARAMS_POT)
+def test_reference_potential_module(n, N, idx, in_golden_dir):
+    import qaxions_amd.fieldtest as field
+    g = np.load("golden_field.npz")
+    assert field.test_potential(n, N, idx) == pytest.approx(g[f"potential_{n}_{N}_{idx}"], abs=1e-12)
+    assert field.test_potential(n, N, idx, fp32=True) == pytest.approx(
+        g[f"potential_{n}_{N}_{idx}"], abs=1e-5, rel=1e-4)
+
+
+# ------------------------------------------------------------------ step compositions
+@pytest.mark.parametrize("dim,N,L", [(2, 8, 1.0), (2, 32, 10.0), (3, 8, 1.0), (3, 16, 20.0)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_leapfrog_matches_oracle(qx, dim, N, L, prec):
+    ds, ns = 2e-3, 7
+    lat = lattice(qx, dim, N, L, prec, pref=1.0)
+    psi = o.ic_grf(dim, N, L, 13)
+    lat.psi = psi
+    lat.update_potential()
+    V, _, _ = o.update_potential(psi, L, 1.0)
+    lat.leapfrog(ds, ns)
+    want, Vw = o.leapfrog(psi, V, L, ds, ns, 1.0)
+    assert rel_l2(lat.psi, want) < (1e-12 if prec == "fp64" else 5e-6)
+    assert rel_l2(lat.V, Vw) < (1e-11 if prec == "fp64" else 5e-5)
+    assert lat.vmax == pytest.approx(np.abs(Vw).max(), rel=(1e-12 if prec == "fp64" else 1e-5))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_leapfrog_segments_and_propagate(qx, prec):
+    dim, N, L, ds = 2, 64, 10.0, 1e-3
+    psi = o.ic_grf(dim, N, L, 17)
+    a = lattice(qx, dim, N, L, prec, pref=1.0)
+    a.psi = psi
+    a.update_potential()
+    a.leapfrog(ds, 6)
+    b = lattice(qx, dim, N, L, prec, pref=1.0)
+    b.psi = psi
+    b.update_potential()
+    b.leapfrog(ds, 2, True, False)
+    b.leapfrog(ds, 4, False, True)
+    assert rel_l2(b.psi, a.psi) < (1e-13 if prec == "fp64" else 1e-6)
+    c = lattice(qx, dim, N, L, prec, pref=1.0)
+    c.psi = psi
+    c.update_potential()
+    for _ in range(6):
+        c.propagate(ds)
+    assert rel_l2(c.psi, a.psi) < (1e-13 if prec == "fp64" else 2e-6)
+    # zero steps: K(ds/2) K(ds/2)
+    d = lattice(qx, dim, N, L, prec, pref=1.0)
+    d.psi = psi
+    d.update_potential()
+    V = d.V
+    d.leapfrog(ds, 0)
+    assert rel_l2(d.psi, o.kick(o.kick(psi, V, ds / 2), V, ds / 2)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_plain_method_256sq_100_steps(qx, prec, tol):
+    """2D 256^2, L = 10, dt = 1e-3, 100 KDK steps, GRF IC: psi relative L2
+    against the FP64 oracle (<= 1e-5 FP32, <= 1e-12 FP64), mass drift and
+    the 64-bin P(k) at the end."""
+    dim, N, L, dt, steps = 2, 256, 10.0, 1e-3, 100
+    psi0 = o.ic_grf(dim, N, L, 0)
+    V0, _, _ = o.update_potential(psi0, L, 1.0)
+    want, _ = o.leapfrog(psi0, V0, L, dt, steps, 1.0)
+    lat = lattice(qx, dim, N, L, prec, pref=1.0)
+    lat.psi = psi0
+    lat.update_potential()
+    lat.leapfrog(dt, steps)
+    got = lat.psi
+    err = rel_l2(got, want)
+    print(f"psi rel L2 after {steps} steps ({prec}): {err:.3e}")
+    assert err <= tol
+    m0 = (np.abs(psi0) ** 2).sum()
+    assert abs((np.abs(got) ** 2).sum() - m0) / m0 < (1e-13 if prec == "fp64" else 1e-6)
+    P, c, _ = lat.spectrum(source="rho", nbins=64, spb=2)
+    Po, co, _ = o.spectrum(np.abs(want) ** 2, L, 64, 2)
+    assert np.array_equal(c, co)
+    assert np.abs(P / Po - 1).max() < (1e-10 if prec == "fp64" else 1e-3)
+
+
+# ------------------------------------------------------------------ initial conditions
+@pytest.mark.parametrize("dim,N", [(2, 64), (3, 16)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_device_ics(qx, dim, N, prec):
+    L = 10.0
+    lat = lattice(qx, dim, N, L, prec)
+    lat.ic_grf(42, 0.1)
+    assert rel_l2(lat.psi, o.ic_grf(dim, N, L, 42, 0.1)) < (1e-12 if prec == "fp64" else 5e-6)
+    tab = o.blob_table(8, 3, L)
+    lat.ic_blobs(tab, 0.1)
+    assert rel_l2(lat.psi, o.ic_blobs(dim, N, L, tab, 0.1)) < (1e-14 if prec == "fp64" else 1e-6)
+    sol = rm.read_solitons()
+    lat.ic_solitons(sol, 1.0)
+    assert rel_l2(lat.psi, o.ic_solitons(dim, N, L, sol, 1.0)) < (1e-14 if prec == "fp64" else 1e-6)
+
+
+# ------------------------------------------------------------------ larger lattices
+@pytest.mark.parametrize("dim,N", [(3, 256), (2, 2048)])
+def test_large_lattice_invariants(qx, dim, N):
+    """Size-independent properties at large N: round trip of the transform,
+    mass conservation through fused steps, determinism."""
+    L = 40.0
+    lat = lattice(qx, dim, N, L, "fp32", pref=1.0)
+    lat.ic_grf(1, 0.1)
+    r0, m0 = lat.density_stats()
+    lat.update_potential()
+    lat.leapfrog(2.5e-4, 5)
+    r1, m1 = lat.density_stats()
+    assert abs(m1 - m0) / m0 < 1e-5
+    e0 = lat.energy()
+    lat.leapfrog(2.5e-4, 5)
+    e1 = lat.energy()
+    assert abs((e1[1] + e1[2]) - (e0[1] + e0[2])) / abs(e0[1] + e0[2]) < 1e-3
