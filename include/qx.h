@@ -143,6 +143,34 @@ const char *qx_step_pass_name(const qx_ctx *ctx, int pass);
  * (nullable) = minimum HBM bytes the pass moves.  cap >= pass count. */
 qx_status qx_time_step_passes(qx_ctx *ctx, double ds, int reps, double *ms, double *bytes, int cap);
 
+/* ---- ensembles of independent 2D lattices (no reference counterpart: the
+ * reference runs one lattice per process) ---------------------------------
+ * Members may have different N (power of two in [16, 4096]) and L; members
+ * of one N form a size class whose state is stored member after member, so
+ * each pass of the step covers the whole class in one launch.  Steady-state
+ * steps are captured in a hipGraph (graph_steps steps per replay, size
+ * classes as parallel branches).  Physics constants are shared; ds is per
+ * member.  Each member evolves exactly as it would alone in a qx_ctx. */
+typedef struct qx_ens qx_ens;
+qx_status qx_ens_create(int nmembers, const int *N, const double *L, qx_precision prec, int device,
+                        qx_ens **out);
+qx_status qx_ens_destroy(qx_ens *ens);
+qx_status qx_ens_set_physics(qx_ens *ens, const qx_physics *ph);
+/* steps per captured graph; 0 launches every pass directly */
+qx_status qx_ens_set_graph_steps(qx_ens *ens, int steps);
+qx_status qx_ens_upload_psi(qx_ens *ens, int member, const double *psi_c128);
+qx_status qx_ens_download_psi(qx_ens *ens, int member, double *psi_c128);
+/* per-member GRF, identical to qx_ic_grf on a lone lattice */
+qx_status qx_ens_ic_grf(qx_ens *ens, const uint64_t *seeds, double amp);
+qx_status qx_ens_update_potential(qx_ens *ens);
+/* qx_leapfrog for every member at once; ds[member]; nsteps >= 1 */
+qx_status qx_ens_leapfrog(qx_ens *ens, const double *ds, int nsteps, int first_half, int last_half);
+/* mass[member] = (L/N)^2 sum |psi|^2 */
+qx_status qx_ens_mass(qx_ens *ens, double *mass);
+int qx_ens_size(const qx_ens *ens);
+int64_t qx_ens_sites(const qx_ens *ens);
+void *qx_ens_stream(qx_ens *ens);
+
 /* ---- plumbing (no reference counterpart) -------------------------------- */
 void *qx_stream(qx_ctx *ctx);          /* the context's hipStream_t */
 qx_status qx_synchronize(qx_ctx *ctx);

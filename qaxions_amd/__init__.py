@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Lattice", "QxError", "lib", "LIB_PATH", "physics_static"]
+__all__ = ["Lattice", "Ensemble", "QxError", "lib", "LIB_PATH", "physics_static"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqx.so")
@@ -80,6 +80,19 @@ SIGNATURES = {
     "qx_step_pass_count": (C.c_int, [C.c_void_p]),
     "qx_step_pass_name": (C.c_char_p, [C.c_void_p, C.c_int]),
     "qx_time_step_passes": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp, _dp, C.c_int]),
+    "qx_ens_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), _dp, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "qx_ens_destroy": (C.c_int, [C.c_void_p]),
+    "qx_ens_set_physics": (C.c_int, [C.c_void_p, C.POINTER(Physics)]),
+    "qx_ens_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "qx_ens_upload_psi": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "qx_ens_download_psi": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "qx_ens_ic_grf": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_double]),
+    "qx_ens_update_potential": (C.c_int, [C.c_void_p]),
+    "qx_ens_leapfrog": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int]),
+    "qx_ens_mass": (C.c_int, [C.c_void_p, _dp]),
+    "qx_ens_size": (C.c_int, [C.c_void_p]),
+    "qx_ens_sites": (C.c_int64, [C.c_void_p]),
+    "qx_ens_stream": (C.c_void_p, [C.c_void_p]),
     "qx_stream": (C.c_void_p, [C.c_void_p]),
     "qx_synchronize": (C.c_int, [C.c_void_p]),
     "qx_last_error": (C.c_char_p, []),
@@ -322,3 +335,74 @@ def blob_table(nblobs, seed, L):
                 L * uniform(seed, st, 2), L * uniform(seed, st, 3), L * uniform(seed, st, 4),
                 2.0 * math.pi * uniform(seed, st, 5)]
     return t
+
+
+class Ensemble:
+    """Independent 2D lattices of uneven sizes advanced together (one launch
+    per size class and pass, hipGraph-captured steps)."""
+
+    def __init__(self, Ns, Ls, precision="fp32", device=0, pref=1.0, rho_bar=1.0, k0_mode=1,
+                 rho_bar_mode=0, graph_steps=8):
+        self.Ns = [int(n) for n in Ns]
+        self.Ls = [float(x) for x in Ls]
+        n = len(self.Ns)
+        h = C.c_void_p()
+        _check(lib().qx_ens_create(n, (C.c_int * n)(*self.Ns), (C.c_double * n)(*self.Ls),
+                                   {"fp32": 0, "fp64": 1}[precision], int(device), C.byref(h)))
+        self._h = h
+        ph = Physics(float(pref), float(rho_bar), int(k0_mode), int(rho_bar_mode))
+        _check(lib().qx_ens_set_physics(self._h, C.byref(ph)))
+        self.set_graph_steps(graph_steps)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qx_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.Ns)
+
+    @property
+    def sites(self):
+        return int(lib().qx_ens_sites(self._h))
+
+    @property
+    def stream(self):
+        return lib().qx_ens_stream(self._h)
+
+    def set_graph_steps(self, k):
+        _check(lib().qx_ens_set_graph_steps(self._h, int(k)))
+
+    def ic_grf(self, seeds, amp=0.1):
+        n = len(self.Ns)
+        _check(lib().qx_ens_ic_grf(self._h, (C.c_uint64 * n)(*[int(x) for x in seeds]), float(amp)))
+
+    def set_psi(self, m, psi):
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert a.shape == (self.Ns[m],) * 2
+        _check(lib().qx_ens_upload_psi(self._h, int(m), _p(a.view(np.float64))))
+
+    def psi(self, m):
+        out = np.empty((self.Ns[m],) * 2, np.complex128)
+        _check(lib().qx_ens_download_psi(self._h, int(m), _p(out.view(np.float64))))
+        return out
+
+    def update_potential(self):
+        _check(lib().qx_ens_update_potential(self._h))
+
+    def leapfrog(self, ds, nsteps, first_half=True, last_half=True):
+        n = len(self.Ns)
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(ds, np.float64), (n,)))
+        _check(lib().qx_ens_leapfrog(self._h, _p(d), int(nsteps), int(bool(first_half)),
+                                     int(bool(last_half))))
+
+    def mass(self):
+        out = np.zeros(len(self.Ns))
+        _check(lib().qx_ens_mass(self._h, _p(out)))
+        return out

@@ -15,10 +15,12 @@ struct Engine {
     virtual void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) = 0;
     virtual void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                          hipStream_t s) = 0;
+    // rows of `rpm` rows per member (rpm = rows for a single lattice)
     virtual void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows,
-                          double pref, double rho_bar, hipStream_t s) = 0;
-    virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, double kick_dt,
-                          int flags, unsigned long long *vmax, hipStream_t s) = 0;
+                          long long rpm, double pref, double rho_bar, hipStream_t s) = 0;
+    virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm,
+                          double kick_dt, const double *kick_member, int flags,
+                          unsigned long long *vmax, hipStream_t s) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
 };
 
@@ -47,8 +49,8 @@ template <int N, typename R> struct EngineT final : Engine {
         else if (op > 0) strided_fft<N, R, +1><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
         else strided_fft<N, R, 0><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
     }
-    void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, double pref,
-                  double rho_bar, hipStream_t s) override
+    void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
+                  double pref, double rho_bar, hipStream_t s) override
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         C *p = (C *)psi;
@@ -56,18 +58,19 @@ template <int N, typename R> struct EngineT final : Engine {
         C *o = (C *)vt;
         const C *t = (const C *)tw;
         if (mode == R2C_REAL)
-            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else if (mode == R2C_DENSITY)
-            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else
-            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
     }
-    void c2r_rows(const void *vt, void *V, void *psi, long long rows, double kick_dt, int flags,
-                  unsigned long long *vmax, hipStream_t s) override
+    void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
+                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s) override
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
-        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows,
-                                                             (R)kick_dt, flags, vmax, (const C *)tw);
+        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
+                                                             (R)kick_dt, kick_member, flags, vmax,
+                                                             (const C *)tw);
     }
 };
 

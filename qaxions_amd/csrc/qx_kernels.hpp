@@ -72,7 +72,30 @@ struct MulArgs {
     double inv_s;    // 1/S
     double pfac;     // Poisson: 1/((2 pi / L)^2 S)
     double k0fac;    // Poisson factor of the k = 0 mode
+    // ensembles: per-member cphase (drift) or pfac (Poisson), member index =
+    // (outer index or line index) / member_div; null for a single lattice
+    const double *member_par;
+    long long member_div;
 };
+
+// the multiply's constants for one member of an ensemble
+__device__ __forceinline__ MulArgs member_args(const MulArgs &a, long long member)
+{
+    MulArgs m = a;
+    if (a.member_par) {
+        if (a.kind == MUL_DRIFT) m.cphase = a.member_par[member];
+        else m.pfac = a.member_par[member];
+    }
+    return m;
+}
+
+// index of (row, k_last) in vt for members of `rpm` rows stored one after
+// another: member-major, then k-major inside a member
+__device__ __forceinline__ long long vt_index(long long row, int k, long long rpm, int h1)
+{
+    const long long mem = row / rpm;
+    return mem * (long long)h1 * rpm + (long long)k * rpm + (row - mem * rpm);
+}
 
 __device__ __forceinline__ int foldi(int i, int N) { return (i <= N / 2) ? i : i - N; }
 
@@ -154,12 +177,13 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
     } else {
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
-        // GEO_VT2: position = k0 (folded), line = k1 (half axis, unfolded)
-        const int k1 = (int)line;
+        // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
+        const int k1 = (int)(line % mul.member_div);
+        const MulArgs ma = member_args(mul, line / mul.member_div);
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int n0 = foldi(t + q * P::TPL, N);
-            v[q] = apply_factor<R>(mul, n0 * n0 + k1 * k1, v[q]);
+            v[q] = apply_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
         }
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
     }
@@ -210,10 +234,11 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
             const int a = foldi((int)col, N);
             mc = a * a + (int)(o * o);
         }
+        const MulArgs ma = member_args(mul, o / mul.member_div);
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int n0 = foldi(t + q * P::TPL, N);
-            v[q] = apply_factor<R>(mul, n0 * n0 + mc, v[q]);
+            v[q] = apply_factor<R>(ma, n0 * n0 + mc, v[q]);
         }
         fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
     }
@@ -236,7 +261,7 @@ enum {
 template <int N, typename R, int MODE>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
-         typename cplx<R>::T *__restrict__ vt, long long rows, R pref, R rho_bar,
+         typename cplx<R>::T *__restrict__ vt, long long rows, long long rpm, R pref, R rho_bar,
          const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
@@ -318,7 +343,7 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __syncthreads();
     for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
         const int k = i / TL, l = i % TL;
-        if (r0 + l < rows) vt[(long long)k * rows + r0 + l] = stg[i];
+        if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[i];
     }
 }
 
@@ -336,7 +361,8 @@ enum {
 template <int N, typename R>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
-         typename cplx<R>::T *__restrict__ psi, long long rows, R kick_dt, int flags,
+         typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
+         const double *__restrict__ kick_member, int flags,
          unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
@@ -357,7 +383,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         C *stg = lds;
         for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
             const int k = i / TL, l = i % TL;
-            stg[i] = (r0 + l < rows) ? vt[(long long)k * rows + r0 + l] : C{0, 0};
+            stg[i] = (r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
         }
         __syncthreads();
         C z[P::E];
@@ -416,11 +442,13 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             b[q] = okb ? psi[rb * N + pos] : C{0, 0};
         }
         if (flags & C2R_KICK) {
+            const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
+            const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 R sa, ca, sb, cb;
-                sincos_r(-kick_dt * va[q], &sa, &ca);
-                sincos_r(-kick_dt * vb[q], &sb, &cb);
+                sincos_r(-dta * va[q], &sa, &ca);
+                sincos_r(-dtb * vb[q], &sb, &cb);
                 a[q] = C{a[q].x * ca - a[q].y * sa, a[q].x * sa + a[q].y * ca};
                 b[q] = C{b[q].x * cb - b[q].y * sb, b[q].x * sb + b[q].y * cb};
             }

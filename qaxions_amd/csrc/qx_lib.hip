@@ -15,6 +15,7 @@
 #include "qx_aux.hpp"
 #include "qx_engine.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -184,6 +185,7 @@ MulArgs drift_args(const qx_ctx *c, double dt)
     m.geo = c->dim == 3 ? GEO_PSI3 : GEO_PSI2;
     m.cphase = -0.5 * dt * dk * dk;
     m.inv_s = 1.0 / (double)c->S;
+    m.member_div = 1;
     return m;
 }
 MulArgs poisson_args(const qx_ctx *c)
@@ -194,6 +196,7 @@ MulArgs poisson_args(const qx_ctx *c)
     m.geo = c->dim == 3 ? GEO_VT3 : GEO_VT2;
     m.pfac = 1.0 / (dk * dk * (double)c->S);
     m.k0fac = (c->ph.k0_mode == 1 && c->ph.rho_bar_mode == 0) ? -1.0 / (double)c->S : 0.0;
+    m.member_div = c->dim == 3 ? 1 : c->H1;
     return m;
 }
 
@@ -282,10 +285,10 @@ qx_status read_vmax(qx_ctx *c)
 qx_status potential_from_psi(qx_ctx *c)
 {
     TRY(reset_vmax(c));
-    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->ph.pref, rho_bar_eff(c),
+    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->rows, c->ph.pref, rho_bar_eff(c),
                      c->stream);
     vt_poisson(c, c->vt);
-    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, 0.0, C2R_SRC_VT | C2R_STORE_V | C2R_VMAX,
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V | C2R_VMAX,
                      c->vmax_bits, c->stream);
     return read_vmax(c);
 }
@@ -328,20 +331,20 @@ void step_pass(qx_ctx *c, int p, double ds, void *psi, void *vt, const MulArgs &
     const long long NN = (long long)N * N;
     if (c->dim == 3) {
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, ds, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
         case 1: c->eng->strided(psi, N, N, -1, none, c->stream); break;
         case 2: c->eng->strided(psi, 1, NN, 0, dm, c->stream); break;
         case 3: c->eng->strided(psi, N, N, +1, none, c->stream); break;
-        case 4: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 4: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
         case 5: c->eng->contig(vt, c->H1 * N, -1, none, c->stream); break;
         case 6: c->eng->strided(vt, c->H1, N, 0, pm, c->stream); break;
         case 7: c->eng->contig(vt, c->H1 * N, +1, none, c->stream); break;
         }
     } else {
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, ds, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
         case 1: c->eng->strided(psi, 1, N, 0, dm, c->stream); break;
-        case 2: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 2: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
     }
@@ -638,9 +641,9 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
         c->spec_cap = c->N / 2 + 1;
     }
     if (source == 0)
-        c->eng->r2c_rows(R2C_REAL, nullptr, c->V, c->aux_vt, c->rows, 1.0, 0.0, c->stream);
+        c->eng->r2c_rows(R2C_REAL, nullptr, c->V, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     else
-        c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->aux_vt, c->rows, 1.0, 0.0, c->stream);
+        c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     vt_forward(c, c->aux_vt);
     HIPCHK(hipMemsetAsync(c->spec_P, 0, sizeof(double) * nbins, c->stream));
     HIPCHK(hipMemsetAsync(c->spec_cnt, 0, sizeof(unsigned long long) * nbins, c->stream));
@@ -682,14 +685,14 @@ qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last
     for (int i = 0; i < nsteps; ++i) {
         // the opening kick reads the stored V; later kicks the potential's spectrum
         if (i == 0)
-            c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, k_first, C2R_KICK | C2R_FWD, nullptr, c->stream);
+            c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->rows, k_first, nullptr, C2R_KICK | C2R_FWD, nullptr, c->stream);
         else
             step_pass(c, 0, ds, c->psi, c->vt, dm, pm);
         for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
     }
     // last step's c2r: materialise V and max|V|, closing half kick
     TRY(reset_vmax(c));
-    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, 0.5 * ds,
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.5 * ds, nullptr,
                      C2R_SRC_VT | C2R_STORE_V | C2R_VMAX | (last_half ? C2R_KICK : 0), c->vmax_bits,
                      c->stream);
     return read_vmax(c);
@@ -784,7 +787,7 @@ qx_status qx_fft_r2c(qx_ctx *c, const double *in, double *out)
     TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
     TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
     TRY(upload_r(c, in, c->aux_r, c->S));
-    c->eng->r2c_rows(R2C_REAL, nullptr, c->aux_r, c->aux_vt, c->rows, 1.0, 0.0, c->stream);
+    c->eng->r2c_rows(R2C_REAL, nullptr, c->aux_r, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     vt_forward(c, c->aux_vt);
     TRY(vt_to_ref_launch(c, c->aux_vt, c->aux_c));
     TRY(sync(c));
@@ -801,7 +804,7 @@ qx_status qx_fft_c2r(qx_ctx *c, const double *in, double *out)
     TRY(upload_c(c, in, c->aux_c, c->H));
     TRY(ref_to_vt_launch(c, c->aux_c, c->aux_vt));
     vt_inverse(c, c->aux_vt);
-    c->eng->c2r_rows(c->aux_vt, c->aux_r, nullptr, c->rows, 0.0, C2R_SRC_VT | C2R_STORE_V, nullptr, c->stream);
+    c->eng->c2r_rows(c->aux_vt, c->aux_r, nullptr, c->rows, c->rows, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V, nullptr, c->stream);
     TRY(sync(c));
     return download_r(c, c->aux_r, out, c->S);
 }
@@ -898,5 +901,446 @@ qx_status qx_synchronize(qx_ctx *c)
     CHECK_CTX(c);
     return sync(c);
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// ensembles of independent 2D lattices (no reference counterpart: the
+// reference runs one lattice per process)
+// ===========================================================================
+namespace {
+
+struct EnsClass {
+    int N = 0, B = 0;
+    long long S = 0, rows = 0, H1 = 0, H = 0;
+    void *psi = nullptr, *V = nullptr, *vt = nullptr;
+    double *d_cphase = nullptr, *d_pfac = nullptr;            // per member
+    double *d_kick_first = nullptr, *d_kick_mid = nullptr, *d_kick_last = nullptr;
+    std::unique_ptr<Engine> eng;
+    std::vector<int> members;   // ensemble member ids, slot order
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+}  // namespace
+
+struct qx_ens {
+    qx_precision prec = QX_FP32;
+    int device = 0, n = 0;
+    size_t csz = 8, rsz = 4;
+    std::vector<int> N, cls, slot;
+    std::vector<double> L, ds_set;
+    std::vector<EnsClass> classes;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    qx_physics ph{1.0, 1.0, 1, 0};
+    int graph_steps = 8;          // steps per captured graph (0: plain launches)
+    hipGraphExec_t gexec = nullptr;
+    int gexec_steps = 0;
+    double *partials = nullptr, *h_partials = nullptr;
+};
+
+namespace {
+
+void ens_free_graph(qx_ens *e)
+{
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+    e->gexec_steps = 0;
+}
+
+// per-member constants for step size ds (array over members)
+qx_status ens_set_ds(qx_ens *e, const double *ds)
+{
+    bool same = (int)e->ds_set.size() == e->n;
+    for (int i = 0; same && i < e->n; ++i) same = e->ds_set[i] == ds[i];
+    if (same) return QX_OK;
+    ens_free_graph(e);
+    e->ds_set.assign(ds, ds + e->n);
+    for (auto &k : e->classes) {
+        std::vector<double> cph(k.B), pf(k.B), kf(k.B), km(k.B), kl(k.B);
+        for (int s = 0; s < k.B; ++s) {
+            const int m = k.members[s];
+            const double dk = TWO_PI / e->L[m];
+            cph[s] = -0.5 * ds[m] * dk * dk;
+            pf[s] = 1.0 / (dk * dk * (double)k.S);
+            km[s] = ds[m];
+            kl[s] = 0.5 * ds[m];
+            kf[s] = 0.5 * ds[m];
+        }
+        HIPCHK(hipMemcpy(k.d_cphase, cph.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(k.d_pfac, pf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(k.d_kick_mid, km.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(k.d_kick_last, kl.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(k.d_kick_first, kf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+    }
+    return QX_OK;
+}
+
+MulArgs ens_drift_args(const EnsClass &k)
+{
+    MulArgs m{};
+    m.kind = MUL_DRIFT;
+    m.geo = GEO_PSI2;
+    m.inv_s = 1.0 / (double)k.S;
+    m.member_par = k.d_cphase;
+    m.member_div = 1;
+    return m;
+}
+MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
+{
+    MulArgs m{};
+    m.kind = MUL_POISSON;
+    m.geo = GEO_VT2;
+    m.k0fac = (e->ph.k0_mode == 1 && e->ph.rho_bar_mode == 0) ? -1.0 / (double)k.S : 0.0;
+    m.member_par = k.d_pfac;
+    m.member_div = k.H1;
+    return m;
+}
+
+// one fused steady-state step of one size class on stream s
+void ens_class_step(qx_ens *e, EnsClass &k, hipStream_t s)
+{
+    const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
+    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_FWD,
+                    nullptr, s);
+    k.eng->strided(k.psi, k.B, k.N, 0, dm, s);
+    k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
+    k.eng->contig(k.vt, k.B * k.H1, 0, pm, s);
+}
+
+// `steps` steady-state steps of every class: classes run as parallel branches
+qx_status ens_launch_steps(qx_ens *e, int steps)
+{
+    for (int i = 0; i < steps; ++i) {
+        HIPCHK(hipEventRecord(e->fork, e->stream));
+        for (auto &k : e->classes) {
+            HIPCHK(hipStreamWaitEvent(k.stream, e->fork, 0));
+            ens_class_step(e, k, k.stream);
+            HIPCHK(hipEventRecord(k.done, k.stream));
+        }
+        for (auto &k : e->classes) HIPCHK(hipStreamWaitEvent(e->stream, k.done, 0));
+    }
+    return QX_OK;
+}
+
+qx_status ens_sync(qx_ens *e)
+{
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return QX_OK;
+}
+
+qx_status ens_steady_steps(qx_ens *e, int steps)
+{
+    if (steps <= 0) return QX_OK;
+    if (e->graph_steps <= 0) return ens_launch_steps(e, steps);
+    const int g = e->graph_steps;
+    if (steps >= g) {
+        if (!e->gexec || e->gexec_steps != g) {
+            ens_free_graph(e);
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            qx_status st = ens_launch_steps(e, g);
+            hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+            if (st != QX_OK) return st;
+            if (ce != hipSuccess) return fail(QX_EDEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+            ce = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ce != hipSuccess) return fail(QX_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ce));
+            e->gexec_steps = g;
+        }
+        for (int i = 0; i + g <= steps; i += g) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    }
+    return ens_launch_steps(e, steps % g);
+}
+
+}  // namespace
+
+extern "C" {
+
+qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision prec, int device,
+                        qx_ens **out)
+{
+    if (!out || n < 1 || !Ns || !Ls) return fail(QX_EINVAL, "need n >= 1 members with sizes and box lengths");
+    *out = nullptr;
+    if (prec != QX_FP32 && prec != QX_FP64) return fail(QX_EINVAL, "unknown precision");
+    for (int i = 0; i < n; ++i) {
+        if (Ns[i] < 16 || (Ns[i] & (Ns[i] - 1)) || Ns[i] > 4096)
+            return fail(QX_EINVAL, "ensemble members must be 2D with N a power of two in [16, 4096]");
+        if (!(Ls[i] > 0.0)) return fail(QX_EINVAL, "L must be positive");
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(QX_EINVAL, "no such device");
+    HIPCHK(hipSetDevice(device));
+    std::unique_ptr<qx_ens> e(new qx_ens());
+    e->prec = prec;
+    e->device = device;
+    e->n = n;
+    e->csz = prec == QX_FP64 ? 16 : 8;
+    e->rsz = prec == QX_FP64 ? 8 : 4;
+    e->N.assign(Ns, Ns + n);
+    e->L.assign(Ls, Ls + n);
+    e->cls.assign(n, -1);
+    e->slot.assign(n, -1);
+    // size classes, ordered by N (work list by size class)
+    std::vector<int> sizes(Ns, Ns + n);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    e->classes.resize(sizes.size());
+    auto cleanup = [&](qx_status s) {
+        qx_ens_destroy(e.release());
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(QX_EDEVICE, "stream creation failed"));
+    for (size_t c = 0; c < sizes.size(); ++c) {
+        EnsClass &k = e->classes[c];
+        k.N = sizes[c];
+        for (int i = 0; i < n; ++i)
+            if (Ns[i] == k.N) {
+                e->cls[i] = (int)c;
+                e->slot[i] = (int)k.members.size();
+                k.members.push_back(i);
+            }
+        k.B = (int)k.members.size();
+        k.S = (long long)k.N * k.N;
+        k.rows = (long long)k.B * k.N;
+        k.H1 = k.N / 2 + 1;
+        k.H = (long long)k.N * k.H1;
+        k.eng = make_engine(k.N, prec == QX_FP64);
+        if (hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(QX_EDEVICE, "stream creation failed"));
+        void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_cphase, (void **)&k.d_pfac,
+                         (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
+        const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
+                             (size_t)(k.B * k.H) * e->csz, (size_t)k.N * e->csz,
+                             sizeof(double) * k.B, sizeof(double) * k.B, sizeof(double) * k.B,
+                             sizeof(double) * k.B, sizeof(double) * k.B};
+        for (int b = 0; b < 9; ++b)
+            if (hipMalloc(bufs[b], sz[b]) != hipSuccess) {
+                *bufs[b] = nullptr;
+                return cleanup(fail(QX_ENOMEM, "ensemble allocation failed"));
+            }
+        std::vector<double> tw(2 * (size_t)k.N);
+        for (int m = 0; m < k.N; ++m) {
+            double a = TWO_PI * (double)m / (double)k.N, cr = std::cos(a), ci = -std::sin(a);
+            if (4 * m == k.N) { cr = 0.0; ci = -1.0; }
+            if (2 * m == k.N) { cr = -1.0; ci = 0.0; }
+            if (4 * m == 3 * k.N) { cr = 0.0; ci = 1.0; }
+            tw[2 * m] = cr;
+            tw[2 * m + 1] = ci;
+        }
+        hipError_t he;
+        if (prec == QX_FP64) {
+            he = hipMemcpy(k.eng->tw, tw.data(), 16 * (size_t)k.N, hipMemcpyHostToDevice);
+        } else {
+            std::vector<float> tf(tw.begin(), tw.end());
+            he = hipMemcpy(k.eng->tw, tf.data(), 8 * (size_t)k.N, hipMemcpyHostToDevice);
+        }
+        if (he != hipSuccess) return cleanup(fail(QX_EDEVICE, "twiddle upload failed"));
+        if (hipMemset(k.psi, 0, sz[0]) != hipSuccess || hipMemset(k.V, 0, sz[1]) != hipSuccess ||
+            hipMemset(k.vt, 0, sz[2]) != hipSuccess)
+            return cleanup(fail(QX_EDEVICE, "state initialisation failed"));
+    }
+    if (hipMalloc((void **)&e->partials, sizeof(double) * 2 * MAX_PART) != hipSuccess ||
+        hipHostMalloc((void **)&e->h_partials, sizeof(double) * 2 * MAX_PART) != hipSuccess)
+        return cleanup(fail(QX_ENOMEM, "ensemble allocation failed"));
+    *out = e.release();
+    return QX_OK;
+}
+
+qx_status qx_ens_destroy(qx_ens *e)
+{
+    if (!e) return QX_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_free_graph(e);
+    for (auto &k : e->classes) {
+        if (k.stream) (void)hipStreamSynchronize(k.stream);
+        void *bufs[] = {k.psi, k.V, k.vt, k.eng ? k.eng->tw : nullptr, k.d_cphase, k.d_pfac,
+                        k.d_kick_first, k.d_kick_mid, k.d_kick_last};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+        if (k.done) (void)hipEventDestroy(k.done);
+        if (k.stream) (void)hipStreamDestroy(k.stream);
+    }
+    if (e->partials) (void)hipFree(e->partials);
+    if (e->h_partials) (void)hipHostFree(e->h_partials);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return QX_OK;
+}
+
+qx_status qx_ens_set_physics(qx_ens *e, const qx_physics *ph)
+{
+    if (!e || !ph) return fail(QX_EINVAL, "null argument");
+    e->ph = *ph;
+    ens_free_graph(e);
+    return QX_OK;
+}
+
+qx_status qx_ens_set_graph_steps(qx_ens *e, int steps)
+{
+    if (!e || steps < 0) return fail(QX_EINVAL, "graph steps must be >= 0");
+    if (steps != e->graph_steps) ens_free_graph(e);
+    e->graph_steps = steps;
+    return QX_OK;
+}
+
+static qx_status ens_member(qx_ens *e, int m, EnsClass **k, long long *off)
+{
+    if (!e || m < 0 || m >= e->n) return fail(QX_EINVAL, "no such member");
+    HIPCHK(hipSetDevice(e->device));
+    *k = &e->classes[e->cls[m]];
+    *off = (long long)e->slot[m] * (*k)->S;
+    return QX_OK;
+}
+
+qx_status qx_ens_upload_psi(qx_ens *e, int m, const double *psi)
+{
+    EnsClass *k;
+    long long off;
+    TRY(ens_member(e, m, &k, &off));
+    if (!psi) return fail(QX_EINVAL, "null psi");
+    if (e->prec == QX_FP64) {
+        HIPCHK(hipMemcpy((double2 *)k->psi + off, psi, (size_t)k->S * 16, hipMemcpyHostToDevice));
+        return QX_OK;
+    }
+    double2 *tmp = nullptr;
+    HIPCHK(hipMalloc((void **)&tmp, (size_t)k->S * 16));
+    hipError_t er = hipMemcpy(tmp, psi, (size_t)k->S * 16, hipMemcpyHostToDevice);
+    if (er == hipSuccess) {
+        c128_to_c<float><<<grid_for(k->S), 256, 0, e->stream>>>(tmp, (float2 *)k->psi + off, k->S);
+        er = hipStreamSynchronize(e->stream);
+    }
+    (void)hipFree(tmp);
+    HIPCHK(er);
+    return QX_OK;
+}
+
+qx_status qx_ens_download_psi(qx_ens *e, int m, double *psi)
+{
+    EnsClass *k;
+    long long off;
+    TRY(ens_member(e, m, &k, &off));
+    if (!psi) return fail(QX_EINVAL, "null psi");
+    if (e->prec == QX_FP64) {
+        HIPCHK(hipMemcpy(psi, (double2 *)k->psi + off, (size_t)k->S * 16, hipMemcpyDeviceToHost));
+        return QX_OK;
+    }
+    double2 *tmp = nullptr;
+    HIPCHK(hipMalloc((void **)&tmp, (size_t)k->S * 16));
+    c_to_c128<float><<<grid_for(k->S), 256, 0, e->stream>>>((const float2 *)k->psi + off, tmp, k->S);
+    hipError_t er = hipStreamSynchronize(e->stream);
+    if (er == hipSuccess) er = hipMemcpy(psi, tmp, (size_t)k->S * 16, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    HIPCHK(er);
+    return QX_OK;
+}
+
+// each member's GRF is generated exactly as a single lattice's (qx_ic_grf)
+qx_status qx_ens_ic_grf(qx_ens *e, const uint64_t *seeds, double amp)
+{
+    if (!e || !seeds) return fail(QX_EINVAL, "null argument");
+    for (int m = 0; m < e->n; ++m) {
+        EnsClass *k;
+        long long off;
+        TRY(ens_member(e, m, &k, &off));
+        qx_ctx *c = nullptr;
+        TRY(qx_create(2, k->N, e->L[m], e->prec, e->device, &c));
+        qx_status st = qx_ic_grf(c, seeds[m], amp);
+        if (st == QX_OK &&
+            hipMemcpy((char *)k->psi + off * e->csz, c->psi, (size_t)k->S * e->csz,
+                      hipMemcpyDeviceToDevice) != hipSuccess)
+            st = fail(QX_EDEVICE, "member copy failed");
+        qx_destroy(c);
+        TRY(st);
+    }
+    return QX_OK;
+}
+
+// P for every member: density, r2c, Poisson, c2r (V stored)
+qx_status qx_ens_update_potential(qx_ens *e)
+{
+    if (!e) return fail(QX_EINVAL, "null ensemble");
+    HIPCHK(hipSetDevice(e->device));
+    for (auto &k : e->classes) {
+        const MulArgs pm = ens_poisson_args(e, k);
+        if (k.d_pfac) {
+            std::vector<double> pf(k.B);
+            for (int s = 0; s < k.B; ++s) {
+                const double dk = TWO_PI / e->L[k.members[s]];
+                pf[s] = 1.0 / (dk * dk * (double)k.S);
+            }
+            HIPCHK(hipMemcpy(k.d_pfac, pf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        }
+        k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
+        k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
+        k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V, nullptr, e->stream);
+    }
+    return ens_sync(e);
+}
+
+qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_half, int last_half)
+{
+    if (!e || !ds) return fail(QX_EINVAL, "null argument");
+    if (nsteps < 1) return fail(QX_EINVAL, "ensemble leapfrog needs nsteps >= 1");
+    HIPCHK(hipSetDevice(e->device));
+    TRY(ens_set_ds(e, ds));
+    // opening kick from the stored V + first step, plain launches on the main stream
+    for (auto &k : e->classes) {
+        std::vector<double> kf(k.B);
+        for (int s = 0; s < k.B; ++s) kf[s] = (first_half ? 0.5 : 1.0) * ds[k.members[s]];
+        HIPCHK(hipMemcpy(k.d_kick_first, kf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
+        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_FWD, nullptr, e->stream);
+        k.eng->strided(k.psi, k.B, k.N, 0, dm, e->stream);
+        k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
+        k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
+    }
+    TRY(ens_steady_steps(e, nsteps - 1));
+    for (auto &k : e->classes)
+        k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_last,
+                        C2R_SRC_VT | C2R_STORE_V | (last_half ? C2R_KICK : 0), nullptr, e->stream);
+    return ens_sync(e);
+}
+
+qx_status qx_ens_mass(qx_ens *e, double *mass)
+{
+    if (!e || !mass) return fail(QX_EINVAL, "null argument");
+    for (int m = 0; m < e->n; ++m) {
+        EnsClass *k;
+        long long off;
+        TRY(ens_member(e, m, &k, &off));
+        const unsigned blocks = grid_for(k->S, RED_THREADS, MAX_PART);
+        if (e->prec == QX_FP64)
+            reduce_kernel<double, RED_RHO><<<blocks, RED_THREADS, 0, e->stream>>>(
+                (const double2 *)k->psi + off, nullptr, k->S, k->N, 2, e->partials);
+        else
+            reduce_kernel<float, RED_RHO><<<blocks, RED_THREADS, 0, e->stream>>>(
+                (const float2 *)k->psi + off, nullptr, k->S, k->N, 2, e->partials);
+        HIPCHK(hipMemcpyAsync(e->h_partials, e->partials, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, e->stream));
+        TRY(ens_sync(e));
+        double s = 0.0;
+        for (unsigned b = 0; b < blocks; ++b) s += e->h_partials[2 * b];
+        mass[m] = s * (e->L[m] / k->N) * (e->L[m] / k->N);
+    }
+    return QX_OK;
+}
+
+int qx_ens_size(const qx_ens *e) { return e ? e->n : 0; }
+int64_t qx_ens_sites(const qx_ens *e)
+{
+    if (!e) return 0;
+    int64_t s = 0;
+    for (int m = 0; m < e->n; ++m) s += (int64_t)e->N[m] * e->N[m];
+    return s;
+}
+void *qx_ens_stream(qx_ens *e) { return e ? (void *)e->stream : nullptr; }
 
 }  // extern "C"

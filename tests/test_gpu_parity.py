@@ -280,3 +280,36 @@ def test_large_lattice_invariants(qx, dim, N):
     lat.leapfrog(2.5e-4, 5)
     e1 = lat.energy()
     assert abs((e1[1] + e1[2]) - (e0[1] + e0[2])) / abs(e0[1] + e0[2]) < 1e-3
+
+
+# ------------------------------------------------------------------ ensembles
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("graph_steps", [0, 3])
+def test_ensemble_members_match_solo_runs(qx, prec, graph_steps):
+    """Each member of a batched ensemble of uneven sizes evolves exactly as
+    the same lattice run alone (same kernels, same constants: bitwise)."""
+    Ns = [16, 64, 16, 32, 64, 16]
+    Ls = [5.0, 12.0, 40.0, 7.5, 20.0, 9.0]
+    seeds = [11, 12, 13, 14, 15, 16]
+    dx0 = 10.0 / 256
+    ds = [1e-3 * (L / N / dx0) ** 2 for N, L in zip(Ns, Ls)]
+    print(f"[case] ensemble prec={prec} graph_steps={graph_steps}", flush=True)
+    ens = qx.Ensemble(Ns, Ls, precision=prec, pref=1.0, graph_steps=graph_steps)
+    ens.ic_grf(seeds)
+    ens.update_potential()
+    ens.leapfrog(ds, 9)
+    m_after = ens.mass()
+    for i, (N, L) in enumerate(zip(Ns, Ls)):
+        solo = qx.Lattice(2, N, L, precision=prec, pref=1.0)
+        solo.ic_grf(seeds[i])
+        solo.update_potential()
+        solo.leapfrog(ds[i], 9)
+        got = ens.psi(i)
+        want = solo.psi
+        assert np.array_equal(got, want), (i, rel_l2(got, want))
+        assert m_after[i] == pytest.approx(solo.density_stats()[1], rel=1e-12)
+    # and against the FP64 oracle for one member
+    psi0 = o.ic_grf(2, Ns[1], Ls[1], seeds[1])
+    V0, _, _ = o.update_potential(psi0, Ls[1], 1.0)
+    want, _ = o.leapfrog(psi0, V0, Ls[1], ds[1], 9, 1.0)
+    assert rel_l2(ens.psi(1), want) < (1e-12 if prec == "fp64" else 5e-6)
