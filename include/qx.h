@@ -65,6 +65,9 @@ qx_status qx_upload_psi(qx_ctx *ctx, const double *psi_c128);
 qx_status qx_download_psi(qx_ctx *ctx, double *psi_c128);
 qx_status qx_upload_V(qx_ctx *ctx, const double *V);
 qx_status qx_download_V(qx_ctx *ctx, double *V);
+/* device-to-device copy of psi, V and the potential spectrum between two
+ * contexts of equal shape and precision (no reference counterpart) */
+qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src);
 /* device pointers of psi / V (element type set by the precision) */
 qx_status qx_device_state(qx_ctx *ctx, void **psi, void **V);
 
@@ -120,6 +123,9 @@ qx_status qx_fft_c2r(qx_ctx *ctx, const double *in_c128, double *out);
 int64_t qx_sites(const qx_ctx *ctx);
 int64_t qx_ksites(const qx_ctx *ctx);
 int qx_global_N(const qx_ctx *ctx);
+/* lattice dimension and state precision of a context (no reference counterpart) */
+int qx_dim(const qx_ctx *ctx);
+qx_precision qx_precision_of(const qx_ctx *ctx);
 
 /* ---- initial conditions on the device (no reference counterpart except
  * qx_ic_solitons, SolitonsIC::apply src/ic/solitons.cxx:20-110) ----------- */
@@ -170,6 +176,39 @@ qx_status qx_ens_mass(qx_ens *ens, double *mass);
 int qx_ens_size(const qx_ens *ens);
 int64_t qx_ens_sites(const qx_ens *ens);
 void *qx_ens_stream(qx_ens *ens);
+
+/* ---- learned coarse-step corrector (no reference counterpart) -------------
+ * Periodic 3x3 convolutions 2 -> 32 -> 32 -> 32 -> 2 with GELU (erf form)
+ * on (Re psi, Im psi) of an H x W lattice: psi <- psi + net(psi).  BF16
+ * weights and hidden activations, FP32 accumulation; the 32 -> 32 forward
+ * layers run on MFMA.  Parameters: 19682 floats, per layer W[co][ci][dy][dx]
+ * then b[co], tap (dy, dx) reading (h + dy - 1, w + dx - 1) mod (H, W).
+ * x/target device arrays are [H*W][2] float (a complex64 psi array). */
+typedef struct qx_net qx_net;
+int qx_net_param_count(void);
+qx_status qx_net_create(int H, int W, int device, qx_net **out);
+qx_status qx_net_destroy(qx_net *net);
+qx_status qx_net_set_params(qx_net *net, const float *theta);  /* resets Adam state */
+qx_status qx_net_get_params(qx_net *net, float *theta);
+qx_status qx_net_forward(qx_net *net, const void *x_dev, void *y_dev);     /* y = x + net(x) */
+qx_status qx_net_forward_host(qx_net *net, const float *x, float *y);
+/* loss = mean((x + net(x) - target)^2) and its gradient (grad_host nullable) */
+qx_status qx_net_loss_grad(qx_net *net, const void *x_dev, const void *t_dev, double *loss, float *grad_host);
+qx_status qx_net_loss_grad_host(qx_net *net, const float *x, const float *t, double *loss, float *grad);
+/* Adam on the last gradient (bias-corrected) */
+qx_status qx_net_adam_step(qx_net *net, float lr, float beta1, float beta2, float eps);
+qx_status qx_net_train_step(qx_net *net, const void *x_dev, const void *t_dev, float lr, double *loss);
+/* psi <- psi + net(psi) on a 2D FP32 context of size H x W (V untouched) */
+qx_status qx_net_correct(qx_net *net, qx_ctx *ctx);
+/* nsteps corrected coarse steps of k*dt each (leapfrog(k dt, 1), correct,
+ * update the potential), stream-ordered with no host waits inside a chunk
+ * of check_every steps; after each chunk the relative mass change is read
+ * back and, beyond mass_tol, the chunk is redone as k*chunk fine steps from
+ * the saved state (*fallbacks = number of redone chunks, nullable) */
+qx_status qx_net_coarse_steps(qx_net *net, qx_ctx *ctx, double dt, int k, int nsteps, int check_every,
+                              double mass_tol, int *fallbacks);
+void *qx_net_stream(qx_net *net);
+const char *qx_net_last_error(void);
 
 /* ---- plumbing (no reference counterpart) -------------------------------- */
 void *qx_stream(qx_ctx *ctx);          /* the context's hipStream_t */

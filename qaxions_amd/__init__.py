@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Lattice", "Ensemble", "QxError", "lib", "LIB_PATH", "physics_static"]
+__all__ = ["Lattice", "Ensemble", "Corrector", "QxError", "lib", "LIB_PATH", "physics_static"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqx.so")
@@ -43,6 +43,7 @@ class Physics(C.Structure):
 
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _i64p = C.POINTER(C.c_int64)
 _lib = None
 
@@ -56,6 +57,7 @@ SIGNATURES = {
     "qx_download_psi": (C.c_int, [C.c_void_p, _dp]),
     "qx_upload_V": (C.c_int, [C.c_void_p, _dp]),
     "qx_download_V": (C.c_int, [C.c_void_p, _dp]),
+    "qx_copy_state": (C.c_int, [C.c_void_p, C.c_void_p]),
     "qx_device_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "qx_kick": (C.c_int, [C.c_void_p, C.c_double]),
     "qx_drift": (C.c_int, [C.c_void_p, C.c_double]),
@@ -93,6 +95,24 @@ SIGNATURES = {
     "qx_ens_size": (C.c_int, [C.c_void_p]),
     "qx_ens_sites": (C.c_int64, [C.c_void_p]),
     "qx_ens_stream": (C.c_void_p, [C.c_void_p]),
+    "qx_dim": (C.c_int, [C.c_void_p]),
+    "qx_precision_of": (C.c_int, [C.c_void_p]),
+    "qx_net_param_count": (C.c_int, []),
+    "qx_net_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "qx_net_destroy": (C.c_int, [C.c_void_p]),
+    "qx_net_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "qx_net_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "qx_net_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qx_net_forward_host": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "qx_net_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _fp]),
+    "qx_net_loss_grad_host": (C.c_int, [C.c_void_p, _fp, _fp, _dp, _fp]),
+    "qx_net_adam_step": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "qx_net_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, _dp]),
+    "qx_net_correct": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qx_net_coarse_steps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.POINTER(C.c_int)]),
+    "qx_net_stream": (C.c_void_p, [C.c_void_p]),
+    "qx_net_last_error": (C.c_char_p, []),
     "qx_stream": (C.c_void_p, [C.c_void_p]),
     "qx_synchronize": (C.c_int, [C.c_void_p]),
     "qx_last_error": (C.c_char_p, []),
@@ -218,6 +238,10 @@ class Lattice:
         if a.shape != self.shape:
             raise ValueError(f"V must have shape {self.shape}")
         _check(lib().qx_upload_V(self._h, _p(a)))
+
+    def copy_from(self, other):
+        """device-to-device copy of psi, V and the potential spectrum"""
+        _check(lib().qx_copy_state(self._h, other.handle))
 
     # -- operators ---------------------------------------------------------------
     def kick(self, dt):
@@ -406,3 +430,80 @@ class Ensemble:
         out = np.zeros(len(self.Ns))
         _check(lib().qx_ens_mass(self._h, _p(out)))
         return out
+
+
+def _check_net(status):
+    if status != QX_OK:
+        raise QxError(status, lib().qx_net_last_error().decode())
+
+
+class Corrector:
+    """Learned coarse-step corrector psi <- psi + net(psi) on H x W lattices."""
+
+    def __init__(self, H, W, device=0, params=None):
+        self.H, self.W = int(H), int(W)
+        h = C.c_void_p()
+        _check_net(lib().qx_net_create(self.H, self.W, int(device), C.byref(h)))
+        self._h = h
+        self.n_params = int(lib().qx_net_param_count())
+        if params is not None:
+            self.params = params
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qx_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check_net(lib().qx_net_get_params(self._h, out.ctypes.data_as(_fp)))
+        return out
+
+    @params.setter
+    def params(self, theta):
+        t = np.ascontiguousarray(theta, np.float32)
+        assert t.size == self.n_params
+        _check_net(lib().qx_net_set_params(self._h, t.ctypes.data_as(_fp)))
+
+    def forward(self, x):
+        """x: [H, W, 2] float array -> x + net(x)"""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty_like(x)
+        _check_net(lib().qx_net_forward_host(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _check_net(lib().qx_net_loss_grad_host(self._h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp),
+                                               C.byref(loss), g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def train_step_lattices(self, coarse, fine, lr):
+        """one Adam step on the pair (coarse.psi -> fine.psi), both FP32 Lattices"""
+        pc, pf = C.c_void_p(), C.c_void_p()
+        _check(lib().qx_device_state(coarse.handle, C.byref(pc), None))
+        _check(lib().qx_device_state(fine.handle, C.byref(pf), None))
+        loss = C.c_double()
+        _check_net(lib().qx_net_train_step(self._h, pc, pf, float(lr), C.byref(loss)))
+        return loss.value
+
+    def correct(self, lat):
+        _check_net(lib().qx_net_correct(self._h, lat.handle))
+
+    def coarse_steps(self, lat, dt, k, nsteps, check_every=10, mass_tol=1e-6):
+        """nsteps corrected coarse steps of k*dt; returns the number of chunks
+        that fell back to fine stepping"""
+        fb = C.c_int()
+        _check_net(lib().qx_net_coarse_steps(self._h, lat.handle, float(dt), int(k), int(nsteps),
+                                             int(check_every), float(mass_tol), C.byref(fb)))
+        return int(fb.value)

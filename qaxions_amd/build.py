@@ -40,7 +40,7 @@ def _run(cmd):
 
 
 def _hip_objects():
-    srcs = ["qx_lib_f32.hip", "qx_lib_f64.hip", "qx_lib.hip"]
+    srcs = ["qx_lib_f32.hip", "qx_lib_f64.hip", "qx_lib.hip", "qx_net.hip"]
     objs = []
     cmds = []
     for s in srcs:

@@ -519,6 +519,19 @@ qx_status qx_download_V(qx_ctx *c, double *h)
     if (!h) return fail(QX_EINVAL, "null V");
     return download_r(c, c->V, h, c->S);
 }
+qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src)
+{
+    if (!valid_ctx(dst) || !valid_ctx(src)) return fail(QX_EINVAL, "null context");
+    if (dst->dim != src->dim || dst->N != src->N || dst->prec != src->prec)
+        return fail(QX_EINVAL, "contexts differ in shape or precision");
+    HIPCHK(hipSetDevice(dst->device));
+    HIPCHK(hipMemcpyAsync(dst->psi, src->psi, (size_t)src->S * src->csz, hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(hipMemcpyAsync(dst->V, src->V, (size_t)src->S * src->rsz, hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(hipMemcpyAsync(dst->vt, src->vt, (size_t)src->H * src->csz, hipMemcpyDeviceToDevice, dst->stream));
+    dst->vmax = src->vmax;
+    return sync(dst);
+}
+
 qx_status qx_device_state(qx_ctx *c, void **psi, void **V)
 {
     if (!valid_ctx(c)) return fail(QX_EINVAL, "null context");
@@ -669,7 +682,7 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
 
 // ---- step compositions --------------------------------------------------------
 
-qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)
+static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half, int last_half, bool wait)
 {
     CHECK_CTX(c);
     if (nsteps < 0) return fail(QX_EINVAL, "nsteps must be >= 0");
@@ -691,12 +704,18 @@ qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last
         for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
     }
     // last step's c2r: materialise V and max|V|, closing half kick
-    TRY(reset_vmax(c));
+    if (wait) TRY(reset_vmax(c));
     c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.5 * ds, nullptr,
-                     C2R_SRC_VT | C2R_STORE_V | C2R_VMAX | (last_half ? C2R_KICK : 0), c->vmax_bits,
-                     c->stream);
-    return read_vmax(c);
+                     C2R_SRC_VT | C2R_STORE_V | (wait ? C2R_VMAX : 0) | (last_half ? C2R_KICK : 0),
+                     c->vmax_bits, c->stream);
+    return wait ? read_vmax(c) : QX_OK;
 }
+
+qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)
+{
+    return leapfrog_impl(c, ds, nsteps, first_half, last_half, true);
+}
+
 
 int qx_step_pass_count(const qx_ctx *c) { return valid_ctx(c) ? step_pass_count(c) : 0; }
 
@@ -810,8 +829,34 @@ qx_status qx_fft_c2r(qx_ctx *c, const double *in, double *out)
 }
 
 int64_t qx_sites(const qx_ctx *c) { return c ? c->S : 0; }
+
+}  // extern "C"
+
+namespace qx {
+// stream-ordered variants without the Vmax read-back (no host wait): used by
+// the corrector to chain solver steps and network passes on one stream
+qx_status leapfrog_enqueue(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)
+{
+    if (nsteps == 0) return fail(QX_EINVAL, "nsteps must be >= 1");
+    return leapfrog_impl(c, ds, nsteps, first_half, last_half, false);
+}
+qx_status update_potential_enqueue(qx_ctx *c)
+{
+    CHECK_CTX(c);
+    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->rows, c->ph.pref, rho_bar_eff(c),
+                     c->stream);
+    vt_poisson(c, c->vt);
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V,
+                     nullptr, c->stream);
+    return QX_OK;
+}
+}  // namespace qx
+
+extern "C" {
 int64_t qx_ksites(const qx_ctx *c) { return c ? c->H : 0; }
 int qx_global_N(const qx_ctx *c) { return c ? c->N : 0; }
+int qx_dim(const qx_ctx *c) { return c ? c->dim : 0; }
+qx_precision qx_precision_of(const qx_ctx *c) { return c ? c->prec : QX_FP32; }
 
 // ---- initial conditions -------------------------------------------------------------
 

@@ -1,0 +1,104 @@
+"""Learned coarse stepping: training pairs from the solver, GPU training of the
+corrector and its evaluation against plain fine and coarse stepping.
+
+A pair is (psi after one coarse KDK step of k*dt, psi after k fine KDK steps
+of dt) from the same state; states are taken along fine trajectories that
+start from Gaussian-random-field ICs.  Everything runs on the device; only
+scalars come back to the host.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import Corrector, Lattice
+
+
+def init_params(seed=0, last_scale=1e-3):
+    """He-style initial parameters; the last layer starts near zero, so the
+    corrector starts close to the identity on psi."""
+    rng = np.random.default_rng(seed)
+    chans = (2, 32, 32, 32, 2)
+    out = []
+    for i, (ci, co) in enumerate(zip(chans[:-1], chans[1:])):
+        w = rng.standard_normal((co, ci, 3, 3)) * np.sqrt(2.0 / (ci * 9))
+        if i == 3:
+            w *= last_scale
+        out += [w.ravel(), np.zeros(co)]
+    return np.concatenate(out).astype(np.float32)
+
+
+def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, log_every=0,
+          pref=1.0):
+    """Train a corrector on len(seeds) * steps_per_seed pairs.  Returns
+    (net, losses)."""
+    if net is None:
+        net = Corrector(N, N, params=init_params())
+    traj = Lattice(2, N, L, "fp32", pref=pref)
+    coarse = Lattice(2, N, L, "fp32", pref=pref)
+    fine = Lattice(2, N, L, "fp32", pref=pref)
+    total = len(seeds) * steps_per_seed
+    losses = []
+    it = 0
+    for seed in seeds:
+        traj.ic_grf(int(seed))
+        traj.update_potential()
+        for _ in range(steps_per_seed):
+            coarse.copy_from(traj)
+            coarse.leapfrog(k * dt, 1)
+            fine.copy_from(traj)
+            fine.leapfrog(dt, k)
+            lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(total - 1, 1))
+            losses.append(net.train_step_lattices(coarse, fine, lr_it))
+            traj.copy_from(fine)
+            it += 1
+            if log_every and it % log_every == 0:
+                print(f"[train] {it}/{total} loss {np.mean(losses[-log_every:]):.3e}", flush=True)
+    for x in (traj, coarse, fine):
+        x.close()
+    return net, np.array(losses)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def evaluate(net, N, L, dt, k, seed, n_coarse, check_every=10, mass_tol=1e-6, ref_refine=4,
+             pref=1.0, timing_reps=1):
+    """Errors after n_coarse coarse intervals (time n_coarse*k*dt) against a
+    reference run with dt/ref_refine, for plain fine stepping (dt), plain
+    coarse stepping (k dt) and corrected coarse stepping; plus device wall
+    times of the fine and corrected runs."""
+    lat0 = Lattice(2, N, L, "fp32", pref=pref)
+    lat0.ic_grf(int(seed))
+    lat0.update_potential()
+    runs = {}
+
+    def run(name, fn):
+        lat = Lattice(2, N, L, "fp32", pref=pref)
+        best = None
+        for _ in range(timing_reps):
+            lat.copy_from(lat0)
+            lat.synchronize()
+            t0 = time.perf_counter()
+            extra = fn(lat)
+            lat.synchronize()
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        runs[name] = (lat.psi, best, extra)
+        lat.close()
+
+    run("reference", lambda lat: lat.leapfrog(dt / ref_refine, n_coarse * k * ref_refine))
+    run("fine", lambda lat: lat.leapfrog(dt, n_coarse * k))
+    run("coarse", lambda lat: lat.leapfrog(k * dt, n_coarse))
+    run("learned", lambda lat: net.coarse_steps(lat, dt, k, n_coarse, check_every, mass_tol))
+    ref = runs["reference"][0]
+    out = {name: {"rel_l2_vs_reference": _rel(p, ref), "seconds": t}
+           for name, (p, t, _) in runs.items() if name != "reference"}
+    out["learned"]["fallback_chunks"] = runs["learned"][2]
+    out["learned"]["rel_l2_vs_fine"] = _rel(runs["learned"][0], runs["fine"][0])
+    out["coarse"]["rel_l2_vs_fine"] = _rel(runs["coarse"][0], runs["fine"][0])
+    out["speedup_learned_vs_fine"] = runs["fine"][1] / runs["learned"][1]
+    lat0.close()
+    return out

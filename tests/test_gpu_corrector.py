@@ -1,0 +1,95 @@
+"""The corrector's HIP forward and backward kernels against the FP64 oracle
+(oracle/corrector.py, with the device's BF16 rounding points), and what the
+trained network gains over coarse stepping without it."""
+import numpy as np
+import pytest
+
+from oracle import corrector as cn
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.mark.parametrize("H,W", [(32, 32), (16, 48), (12, 12), (8, 20)])
+def test_forward_matches_oracle(qx, H, W):
+    """W % 16 == 0 runs the MFMA path for the 32 -> 32 layers, else the vector path."""
+    print(f"[case] corrector forward {H}x{W}", flush=True)
+    rng = np.random.default_rng(H * 100 + W)
+    theta = cn.init_params(5, last_scale=0.5)
+    x = rng.standard_normal((H, W, 2))
+    net = qx.Corrector(H, W, params=theta.astype(np.float32))
+    y = net.forward(x.astype(np.float32)).astype(np.float64)
+    theta32 = theta.astype(np.float32).astype(np.float64)
+    want = x.astype(np.float32).astype(np.float64) + cn.forward(theta32, x.astype(np.float32).astype(np.float64), bf16=True)
+    # compare the network part (y - x), where rounding points decide agreement
+    assert rel(y - x, want - x) < 1e-2
+
+
+@pytest.mark.parametrize("H,W", [(16, 16), (12, 20)])
+def test_gradient_matches_oracle(qx, H, W):
+    print(f"[case] corrector gradient {H}x{W}", flush=True)
+    rng = np.random.default_rng(7)
+    theta = cn.init_params(9, last_scale=0.5).astype(np.float32)
+    x = rng.standard_normal((H, W, 2)).astype(np.float32)
+    t = (x + 0.1 * rng.standard_normal((H, W, 2))).astype(np.float32)
+    net = qx.Corrector(H, W, params=theta)
+    loss, g = net.loss_grad(x, t)
+    lo, go = cn.loss_and_grad(theta.astype(np.float64), x.astype(np.float64), t.astype(np.float64), bf16=True)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    assert rel(g, go) < 2e-2
+    # every layer's weights and biases individually
+    o = 0
+    for (ws, bs) in cn.param_shapes():
+        nw, nb = int(np.prod(ws)), int(np.prod(bs))
+        assert rel(g[o:o + nw], go[o:o + nw]) < 5e-2
+        assert rel(g[o + nw:o + nw + nb], go[o + nw:o + nw + nb]) < 5e-2
+        o += nw + nb
+
+
+def test_adam_step_lowers_loss(qx):
+    rng = np.random.default_rng(3)
+    H = W = 16
+    x = rng.standard_normal((H, W, 2)).astype(np.float32)
+    t = (x + 0.05 * np.roll(x, 1, axis=0)).astype(np.float32)
+    net = qx.Corrector(H, W, params=cn.init_params(1).astype(np.float32))
+    l0, _ = net.loss_grad(x, t)
+    for _ in range(30):
+        net.loss_grad(x, t)
+        qx.lib().qx_net_adam_step(net._h, 1e-3, 0.9, 0.999, 1e-8)
+    l1, _ = net.loss_grad(x, t)
+    assert l1 < 0.5 * l0
+
+
+def test_forced_fallback_equals_fine_steps(qx):
+    """mass_tol < 0 forces every chunk back to fine stepping: the result is
+    exactly k*n fine steps (the failure path, forced from the host)."""
+    from qaxions_amd.learned import init_params
+    N, L, dt, k = 64, 10.0, 1e-3, 4
+    net = qx.Corrector(N, N, params=init_params())
+    a = qx.Lattice(2, N, L, "fp32", pref=1.0)
+    a.ic_grf(5)
+    a.update_potential()
+    b = qx.Lattice(2, N, L, "fp32", pref=1.0)
+    b.copy_from(a)
+    fb = net.coarse_steps(a, dt, k, 6, check_every=3, mass_tol=-1.0)
+    assert fb == 2
+    b.leapfrog(dt, k * 3)
+    b.leapfrog(dt, k * 3)
+    assert np.array_equal(a.psi, b.psi)
+
+
+def test_trained_corrector_beats_plain_coarse(qx):
+    """Trained on solver pairs (seeds disjoint from evaluation), corrected
+    coarse stepping is closer to the fine solution than coarse stepping alone."""
+    from qaxions_amd import learned
+    N, L, dt, k = 64, 10.0, 4e-3, 4
+    net, losses = learned.train(N, L, dt, k, seeds=range(1000, 1008), steps_per_seed=40, lr=2e-3,
+                                lr_final=2e-4)
+    assert losses[-40:].mean() < 0.5 * losses[:40].mean()
+    res = learned.evaluate(net, N, L, dt, k, seed=1, n_coarse=20, check_every=5, mass_tol=1e-3)
+    print(res)
+    assert res["learned"]["rel_l2_vs_fine"] < 0.7 * res["coarse"]["rel_l2_vs_fine"]
+    assert res["learned"]["rel_l2_vs_reference"] < res["coarse"]["rel_l2_vs_reference"]
