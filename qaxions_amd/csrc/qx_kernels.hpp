@@ -68,12 +68,16 @@ enum {
 struct MulArgs {
     int kind;        // MUL_*
     int geo;         // GEO_*
-    double cphase;   // drift: -dt/2 (2 pi / L)^2
+    // drift: per-axis FP64 phase table ptab[i] = e^{-i dt/2 (2 pi/L)^2 fold(i)^2}
+    // (N entries per member, members ptab_stride apart); the factor of a mode is
+    // the product over its axes, formed in FP64 and rounded once
+    const double2 *ptab;
+    long long ptab_stride;
     double inv_s;    // 1/S
     double pfac;     // Poisson: 1/((2 pi / L)^2 S)
     double k0fac;    // Poisson factor of the k = 0 mode
-    // ensembles: per-member cphase (drift) or pfac (Poisson), member index =
-    // (outer index or line index) / member_div; null for a single lattice
+    // ensembles: per-member pfac (Poisson), member index = (outer index or
+    // line index) / member_div; null for a single lattice
     const double *member_par;
     long long member_div;
 };
@@ -82,10 +86,8 @@ struct MulArgs {
 __device__ __forceinline__ MulArgs member_args(const MulArgs &a, long long member)
 {
     MulArgs m = a;
-    if (a.member_par) {
-        if (a.kind == MUL_DRIFT) m.cphase = a.member_par[member];
-        else m.pfac = a.member_par[member];
-    }
+    if (a.kind == MUL_DRIFT) m.ptab = a.ptab + member * a.ptab_stride;
+    else if (a.member_par) m.pfac = a.member_par[member];
     return m;
 }
 
@@ -99,47 +101,50 @@ __device__ __forceinline__ long long vt_index(long long row, int k, long long rp
 
 __device__ __forceinline__ int foldi(int i, int N) { return (i <= N / 2) ? i : i - N; }
 
-// e^{i cphase m} / S.  The argument is formed and reduced mod 2 pi in FP64
-// (Cody-Waite, two-word 2 pi) so that high-k phases keep full FP32 accuracy.
-template <typename R> struct Phase;
-template <> struct Phase<float> {
-    static __device__ __forceinline__ float2 factor(double cphase, int m, double inv_s)
-    {
-        const double th = cphase * (double)m;
-        const double kk = rint(th * 0.15915494309189533577);
-        double r = fma(-kk, 6.283185307179586232, th);
-        r = fma(-kk, 2.4492935982947064e-16, r);
-        float s, c;
-        sincosf((float)r, &s, &c);
-        const float f = (float)inv_s;
-        return float2{c * f, s * f};
-    }
-};
-template <> struct Phase<double> {
-    static __device__ __forceinline__ double2 factor(double cphase, int m, double inv_s)
-    {
-        const double th = cphase * (double)m;
-        const double kk = rint(th * 0.15915494309189533577);
-        double r = fma(-kk, 6.283185307179586232, th);
-        r = fma(-kk, 2.4492935982947064e-16, r);
-        double s, c;
-        sincos(r, &s, &c);
-        return double2{c * inv_s, s * inv_s};
-    }
-};
-
 __device__ __forceinline__ void sincos_r(float x, float *s, float *c) { sincosf(x, s, c); }
 __device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sincos(x, s, c); }
 
-template <typename R, typename C>
-__device__ __forceinline__ C apply_factor(const MulArgs &a, int m, C x)
+// sin/cos of a kick angle.  |x| <= pi/4 (always so at the step sizes the
+// method is run with): Taylor polynomials to x^9 / x^10, truncation error
+// below 3e-9 relative; otherwise the library sincosf.
+__device__ __forceinline__ void sincos_kick(float x, float *s, float *c)
 {
-    if (a.kind == MUL_DRIFT) {
-        C f = Phase<R>::factor(a.cphase, m, a.inv_s);
-        return cmul(x, f);
+    if (fabsf(x) <= 0.78539816f) {
+        const float x2 = x * x;
+        *s = x * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f),
+                               -1.6666667e-1f), 1.0f);
+        *c = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
+                                    4.1666667e-2f), -0.5f), 1.0f);
+    } else {
+        sincosf(x, s, c);
     }
-    R f = (m == 0) ? (R)a.k0fac : -(R)a.pfac / (R)m;
+}
+__device__ __forceinline__ void sincos_kick(double x, double *s, double *c) { sincos(x, s, c); }
+
+__device__ __forceinline__ float recip(float m) { return __builtin_amdgcn_rcpf(m); }
+__device__ __forceinline__ double recip(double m) { return 1.0 / m; }
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return double2{fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x)};
+}
+
+// Poisson factor -1/(k^2 S) for integer m = |n|^2 (k0fac at m = 0)
+template <typename R, typename C>
+__device__ __forceinline__ C poisson_factor(const MulArgs &a, int m, C x)
+{
+    const R f = (m == 0) ? (R)a.k0fac : -(R)a.pfac * recip((R)m);
     return C{x.x * f, x.y * f};
+}
+
+// drift factor: table entry of the line position times the column factor
+// (both FP64), rounded once to the state precision
+template <typename C>
+__device__ __forceinline__ C drift_factor(const double2 &tp, const double2 &cf, C x)
+{
+    const double2 f = zmul(tp, cf);
+    using Rt = decltype(x.x);
+    return cmul(x, C{(Rt)f.x, (Rt)f.y});
 }
 
 template <int N, typename C>
@@ -183,7 +188,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int n0 = foldi(t + q * P::TPL, N);
-            v[q] = apply_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
+            v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
         }
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
     }
@@ -223,22 +228,25 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
     } else {
         fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-        int mc;
-        if (mul.geo == GEO_PSI3) {
-            const int a = foldi((int)(col / N), N), b = foldi((int)(col % N), N);
-            mc = a * a + b * b;
-        } else if (mul.geo == GEO_PSI2) {
-            const int a = foldi((int)col, N);
-            mc = a * a;
-        } else {  // GEO_VT3
-            const int a = foldi((int)col, N);
-            mc = a * a + (int)(o * o);
-        }
         const MulArgs ma = member_args(mul, o / mul.member_div);
+        const int cc = ok ? (int)col : 0;
+        if (mul.kind == MUL_DRIFT) {
+            const double2 *T = ma.ptab;
+            double2 cf;
+            if (mul.geo == GEO_PSI3) cf = zmul(T[cc / N], T[cc % N]);
+            else cf = T[cc];
+            cf.x *= ma.inv_s;
+            cf.y *= ma.inv_s;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int n0 = foldi(t + q * P::TPL, N);
-            v[q] = apply_factor<R>(ma, n0 * n0 + mc, v[q]);
+            for (int q = 0; q < P::E; ++q) v[q] = drift_factor(T[t + q * P::TPL], cf, v[q]);
+        } else {  // GEO_VT3 Poisson
+            const int a = foldi(cc, N);
+            const int mc = a * a + (int)(o * o);
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int n0 = foldi(t + q * P::TPL, N);
+                v[q] = poisson_factor<R>(ma, n0 * n0 + mc, v[q]);
+            }
         }
         fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
     }
@@ -447,8 +455,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 R sa, ca, sb, cb;
-                sincos_r(-dta * va[q], &sa, &ca);
-                sincos_r(-dtb * vb[q], &sb, &cb);
+                sincos_kick(-dta * va[q], &sa, &ca);
+                sincos_kick(-dtb * vb[q], &sb, &cb);
                 a[q] = C{a[q].x * ca - a[q].y * sa, a[q].x * sa + a[q].y * ca};
                 b[q] = C{b[q].x * cb - b[q].y * sb, b[q].x * sb + b[q].y * cb};
             }

@@ -78,6 +78,8 @@ struct qx_ctx {
     hipStream_t stream = nullptr;
     qx_physics ph{4.0 * M_PI, 1.0, 1, 0};
     double vmax = 0.0;
+    double2 *ptab = nullptr;   // drift phase table (N entries)
+    double ptab_dt = std::nan("");
     std::unique_ptr<Engine> eng;
 };
 
@@ -177,16 +179,36 @@ qx_status download_r(qx_ctx *c, const void *d, double *h, long long n)
     return sync(c);
 }
 
-MulArgs drift_args(const qx_ctx *c, double dt)
+// per-axis drift phase table T[i] = e^{-i dt/2 (2 pi/L)^2 n^2}, n = fold(i),
+// evaluated in FP64 on the host (libm reduces the argument exactly)
+void fill_phase_table(double2 *T, int N, double L, double dt)
 {
-    const double dk = TWO_PI / c->L;
-    MulArgs m{};
-    m.kind = MUL_DRIFT;
-    m.geo = c->dim == 3 ? GEO_PSI3 : GEO_PSI2;
-    m.cphase = -0.5 * dt * dk * dk;
-    m.inv_s = 1.0 / (double)c->S;
-    m.member_div = 1;
-    return m;
+    const double dk = TWO_PI / L, c = -0.5 * dt * dk * dk;
+    for (int i = 0; i < N; ++i) {
+        const double n = (i <= N / 2) ? i : i - N;
+        const double th = c * n * n;
+        T[i] = double2{std::cos(th), std::sin(th)};
+    }
+}
+
+qx_status drift_args(qx_ctx *c, double dt, MulArgs *m)
+{
+    if (!c->ptab) TRY(ensure(c, (void **)&c->ptab, sizeof(double2) * c->N));
+    if (!(c->ptab_dt == dt)) {
+        std::vector<double2> T(c->N);
+        fill_phase_table(T.data(), c->N, c->L, dt);
+        HIPCHK(hipStreamSynchronize(c->stream));  // no kernel may still read the old table
+        HIPCHK(hipMemcpy(c->ptab, T.data(), sizeof(double2) * c->N, hipMemcpyHostToDevice));
+        c->ptab_dt = dt;
+    }
+    *m = MulArgs{};
+    m->kind = MUL_DRIFT;
+    m->geo = c->dim == 3 ? GEO_PSI3 : GEO_PSI2;
+    m->ptab = c->ptab;
+    m->ptab_stride = 0;
+    m->inv_s = 1.0 / (double)c->S;
+    m->member_div = 1;
+    return QX_OK;
 }
 MulArgs poisson_args(const qx_ctx *c)
 {
@@ -214,11 +236,12 @@ void c2c_passes(qx_ctx *c, void *data, int sign)
     }
 }
 // the two (3D) or one (2D) leading-axis passes of psi around the drift
-void drift_core(qx_ctx *c, double dt)
+qx_status drift_core(qx_ctx *c, double dt)
 {
     MulArgs none{};
     const int N = c->N;
-    const MulArgs m = drift_args(c, dt);
+    MulArgs m;
+    TRY(drift_args(c, dt, &m));
     if (c->dim == 3) {
         c->eng->strided(c->psi, N, N, -1, none, c->stream);
         c->eng->strided(c->psi, 1, (long long)N * N, 0, m, c->stream);
@@ -226,6 +249,7 @@ void drift_core(qx_ctx *c, double dt)
     } else {
         c->eng->strided(c->psi, 1, N, 0, m, c->stream);
     }
+    return QX_OK;
 }
 void vt_forward(qx_ctx *c, void *vt)
 {
@@ -469,7 +493,7 @@ qx_status qx_destroy(qx_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->psi, c->V, c->vt, c->aux_c, c->aux_vt, c->aux_r, c->stage,
-                    c->partials, c->vmax_bits, c->spec_P, c->spec_cnt,
+                    c->partials, c->vmax_bits, c->spec_P, c->spec_cnt, c->ptab,
                     c->eng ? c->eng->tw : nullptr};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -558,7 +582,7 @@ qx_status qx_drift(qx_ctx *c, double dt)
     CHECK_CTX(c);
     MulArgs none{};
     c->eng->contig(c->psi, c->rows, -1, none, c->stream);
-    drift_core(c, dt);
+    TRY(drift_core(c, dt));
     c->eng->contig(c->psi, c->rows, +1, none, c->stream);
     return sync(c);
 }
@@ -693,7 +717,8 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
         return QX_OK;
     }
     const MulArgs pm = poisson_args(c);
-    const MulArgs dm = drift_args(c, ds);
+    MulArgs dm;
+    TRY(drift_args(c, ds, &dm));
     const int np = step_pass_count(c);
     for (int i = 0; i < nsteps; ++i) {
         // the opening kick reads the stored V; later kicks the potential's spectrum
@@ -738,7 +763,8 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     HIPCHK(hipMemcpyAsync(c->aux_c, c->psi, (size_t)c->S * c->csz, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->aux_vt, c->vt, (size_t)c->H * c->csz, hipMemcpyDeviceToDevice, c->stream));
     const MulArgs pm = poisson_args(c);
-    const MulArgs dm = drift_args(c, ds);
+    MulArgs dm;
+    TRY(drift_args(c, ds, &dm));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -959,7 +985,8 @@ struct EnsClass {
     int N = 0, B = 0;
     long long S = 0, rows = 0, H1 = 0, H = 0;
     void *psi = nullptr, *V = nullptr, *vt = nullptr;
-    double *d_cphase = nullptr, *d_pfac = nullptr;            // per member
+    double2 *d_ptab = nullptr;                                 // per member phase tables [B][N]
+    double *d_pfac = nullptr;                                  // per member
     double *d_kick_first = nullptr, *d_kick_mid = nullptr, *d_kick_last = nullptr;
     std::unique_ptr<Engine> eng;
     std::vector<int> members;   // ensemble member ids, slot order
@@ -1003,17 +1030,18 @@ qx_status ens_set_ds(qx_ens *e, const double *ds)
     ens_free_graph(e);
     e->ds_set.assign(ds, ds + e->n);
     for (auto &k : e->classes) {
-        std::vector<double> cph(k.B), pf(k.B), kf(k.B), km(k.B), kl(k.B);
+        std::vector<double> pf(k.B), kf(k.B), km(k.B), kl(k.B);
+        std::vector<double2> tab((size_t)k.B * k.N);
         for (int s = 0; s < k.B; ++s) {
             const int m = k.members[s];
             const double dk = TWO_PI / e->L[m];
-            cph[s] = -0.5 * ds[m] * dk * dk;
+            fill_phase_table(tab.data() + (size_t)s * k.N, k.N, e->L[m], ds[m]);
             pf[s] = 1.0 / (dk * dk * (double)k.S);
             km[s] = ds[m];
             kl[s] = 0.5 * ds[m];
             kf[s] = 0.5 * ds[m];
         }
-        HIPCHK(hipMemcpy(k.d_cphase, cph.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(k.d_ptab, tab.data(), sizeof(double2) * tab.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(k.d_pfac, pf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(k.d_kick_mid, km.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(k.d_kick_last, kl.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
@@ -1028,7 +1056,8 @@ MulArgs ens_drift_args(const EnsClass &k)
     m.kind = MUL_DRIFT;
     m.geo = GEO_PSI2;
     m.inv_s = 1.0 / (double)k.S;
-    m.member_par = k.d_cphase;
+    m.ptab = k.d_ptab;
+    m.ptab_stride = k.N;
     m.member_div = 1;
     return m;
 }
@@ -1159,11 +1188,11 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
         if (hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)
             return cleanup(fail(QX_EDEVICE, "stream creation failed"));
-        void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_cphase, (void **)&k.d_pfac,
+        void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_ptab, (void **)&k.d_pfac,
                          (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
         const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
                              (size_t)(k.B * k.H) * e->csz, (size_t)k.N * e->csz,
-                             sizeof(double) * k.B, sizeof(double) * k.B, sizeof(double) * k.B,
+                             sizeof(double2) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
                              sizeof(double) * k.B, sizeof(double) * k.B};
         for (int b = 0; b < 9; ++b)
             if (hipMalloc(bufs[b], sz[b]) != hipSuccess) {
@@ -1206,7 +1235,7 @@ qx_status qx_ens_destroy(qx_ens *e)
     ens_free_graph(e);
     for (auto &k : e->classes) {
         if (k.stream) (void)hipStreamSynchronize(k.stream);
-        void *bufs[] = {k.psi, k.V, k.vt, k.eng ? k.eng->tw : nullptr, k.d_cphase, k.d_pfac,
+        void *bufs[] = {k.psi, k.V, k.vt, k.eng ? k.eng->tw : nullptr, k.d_ptab, k.d_pfac,
                         k.d_kick_first, k.d_kick_mid, k.d_kick_last};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
