@@ -179,7 +179,7 @@ void *qx_ens_stream(qx_ens *ens);
 
 /* ---- learned coarse-step corrector (no reference counterpart) -------------
  * Periodic 3x3 convolutions 2 -> 32 -> 32 -> 32 -> 2 with GELU (erf form)
- * on (Re psi, Im psi) of an H x W lattice: psi <- psi + net(psi).  BF16
+ * on (Re psi, Im psi) of an H x W lattice: psi <- psi + s net(psi).  BF16
  * weights and hidden activations, FP32 accumulation; the 32 -> 32 forward
  * layers run on MFMA.  Parameters: 19682 floats, per layer W[co][ci][dy][dx]
  * then b[co], tap (dy, dx) reading (h + dy - 1, w + dx - 1) mod (H, W).
@@ -189,6 +189,10 @@ int qx_net_param_count(void);
 qx_status qx_net_create(int H, int W, int device, qx_net **out);
 qx_status qx_net_destroy(qx_net *net);
 qx_status qx_net_set_params(qx_net *net, const float *theta);  /* resets Adam state */
+/* fixed output scale s: y = x + s net(x) (default 1); set it to the typical
+ * size of the correction so the network works with O(1) outputs */
+qx_status qx_net_set_output_scale(qx_net *net, float s);
+float qx_net_output_scale(const qx_net *net);
 qx_status qx_net_get_params(qx_net *net, float *theta);
 qx_status qx_net_forward(qx_net *net, const void *x_dev, void *y_dev);     /* y = x + net(x) */
 qx_status qx_net_forward_host(qx_net *net, const float *x, float *y);

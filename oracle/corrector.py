@@ -127,16 +127,16 @@ def forward(theta, x, keep=False, bf16=False):
     return (a, cache) if keep else a
 
 
-def loss_and_grad(theta, x, target, bf16=False):
-    """loss = mean((x + net(x) - target)^2); gradient w.r.t. theta (with
-    bf16=True the rounding is treated as the identity in the backward pass,
-    as the device kernels do)"""
+def loss_and_grad(theta, x, target, bf16=False, scale=1.0):
+    """loss = mean((x + scale net(x) - target)^2); gradient w.r.t. theta
+    (with bf16=True the rounding is treated as the identity in the backward
+    pass, as the device kernels do)"""
     layers = _layers(theta, bf16)
     out, cache = forward(theta, x, keep=True, bf16=bf16)
-    r = x + out - target
+    r = x + scale * out - target
     n = r.size
     loss = float((r * r).sum() / n)
-    d = 2.0 * r / n
+    d = 2.0 * scale * r / n
     grads = []
     for i in range(len(layers) - 1, -1, -1):
         a_in, z = cache[i]

@@ -102,6 +102,8 @@ SIGNATURES = {
     "qx_net_destroy": (C.c_int, [C.c_void_p]),
     "qx_net_set_params": (C.c_int, [C.c_void_p, _fp]),
     "qx_net_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "qx_net_set_output_scale": (C.c_int, [C.c_void_p, C.c_float]),
+    "qx_net_output_scale": (C.c_float, [C.c_void_p]),
     "qx_net_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "qx_net_forward_host": (C.c_int, [C.c_void_p, _fp, _fp]),
     "qx_net_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _fp]),
@@ -472,8 +474,16 @@ class Corrector:
         assert t.size == self.n_params
         _check_net(lib().qx_net_set_params(self._h, t.ctypes.data_as(_fp)))
 
+    @property
+    def output_scale(self):
+        return float(lib().qx_net_output_scale(self._h))
+
+    @output_scale.setter
+    def output_scale(self, s):
+        _check_net(lib().qx_net_set_output_scale(self._h, float(s)))
+
     def forward(self, x):
-        """x: [H, W, 2] float array -> x + net(x)"""
+        """x: [H, W, 2] float array -> x + s net(x)"""
         x = np.ascontiguousarray(x, np.float32)
         y = np.empty_like(x)
         _check_net(lib().qx_net_forward_host(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp)))

@@ -53,10 +53,9 @@ __global__ void kick_kernel(typename cplx<R>::T *__restrict__ psi, const R *__re
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        R s, c;
-        sincos_r(-dt * V[i], &s, &c);
-        auto a = psi[i];
-        psi[i] = typename cplx<R>::T{a.x * c - a.y * s, a.x * s + a.y * c};
+        R s, cm1;
+        sincosm1_kick(-dt * V[i], &s, &cm1);
+        psi[i] = rotate(psi[i], s, cm1);
     }
 }
 

@@ -60,11 +60,18 @@ template <int DIR, typename C> struct Dft<4, DIR, C> {
         a[3] = csub(t1, t3);
     }
 };
+// multiplication by 1/sqrt(2).  In FP32 the constant is split in two
+// words: the rounded constant alone has 2h^2 = 1 - 4.1e-8, which would
+// shrink every W8 rotation and bias the norm (mass) of psi step after step.
+__device__ __forceinline__ float mul_rsqrt2(float a)
+{
+    return fmaf(0.70710676908493042f, a, 1.2101617e-08f * a);
+}
+__device__ __forceinline__ double mul_rsqrt2(double a) { return 0.70710678118654752440 * a; }
+
 template <int DIR, typename C> struct Dft<8, DIR, C> {
     static __device__ __forceinline__ void run(C *a)
     {
-        using Rt = decltype(a[0].x);
-        const Rt h = (Rt)0.70710678118654752440084436210485;
         C e[4] = {a[0], a[2], a[4], a[6]};
         C o[4] = {a[1], a[3], a[5], a[7]};
         Dft<4, DIR, C>::run(e);
@@ -72,11 +79,11 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
         // twiddle the odd half by W8^k
         C o1, o3;
         if (DIR < 0) {
-            o1 = C{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
-            o3 = C{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+            o1 = C{mul_rsqrt2(o[1].x + o[1].y), mul_rsqrt2(o[1].y - o[1].x)};
+            o3 = C{mul_rsqrt2(o[3].y - o[3].x), -mul_rsqrt2(o[3].x + o[3].y)};
         } else {
-            o1 = C{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)};
-            o3 = C{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)};
+            o1 = C{mul_rsqrt2(o[1].x - o[1].y), mul_rsqrt2(o[1].x + o[1].y)};
+            o3 = C{-mul_rsqrt2(o[3].x + o[3].y), mul_rsqrt2(o[3].x - o[3].y)};
         }
         C o2 = mul_i<DIR>(o[2]);
         a[0] = cadd(e[0], o[0]);

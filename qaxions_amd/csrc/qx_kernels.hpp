@@ -104,22 +104,39 @@ __device__ __forceinline__ int foldi(int i, int N) { return (i <= N / 2) ? i : i
 __device__ __forceinline__ void sincos_r(float x, float *s, float *c) { sincosf(x, s, c); }
 __device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sincos(x, s, c); }
 
-// sin/cos of a kick angle.  |x| <= pi/4 (always so at the step sizes the
-// method is run with): Taylor polynomials to x^9 / x^10, truncation error
-// below 3e-9 relative; otherwise the library sincosf.
-__device__ __forceinline__ void sincos_kick(float x, float *s, float *c)
+// sin x and cos x - 1 of a kick angle.  |x| <= pi/4 (always so at the step
+// sizes the method is run with): Taylor polynomials to x^9 / x^10, truncation
+// error below 3e-9 relative; otherwise the library sincosf.  cos x - 1 is
+// returned directly so that psi + psi (e^{ix} - 1) rounds once, unbiased:
+// rounding cos x itself near 1 would make |e^{ix}| exceed 1 on average and
+// drift the mass.
+__device__ __forceinline__ void sincosm1_kick(float x, float *s, float *cm1)
 {
     if (fabsf(x) <= 0.78539816f) {
         const float x2 = x * x;
         *s = x * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f),
                                -1.6666667e-1f), 1.0f);
-        *c = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
-                                    4.1666667e-2f), -0.5f), 1.0f);
+        *cm1 = x2 * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
+                                   4.1666667e-2f), -0.5f);
     } else {
-        sincosf(x, s, c);
+        float c, sh = sinf(0.5f * x);
+        sincosf(x, s, &c);
+        *cm1 = -2.0f * sh * sh;
     }
 }
-__device__ __forceinline__ void sincos_kick(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void sincosm1_kick(double x, double *s, double *cm1)
+{
+    double c, sh = sin(0.5 * x);
+    sincos(x, s, &c);
+    *cm1 = -2.0 * sh * sh;
+}
+// psi * e^{ix} = psi + psi (e^{ix} - 1)
+template <typename C, typename R>
+__device__ __forceinline__ C rotate(C a, R s, R cm1)
+{
+    const R tr = fma(a.x, cm1, -a.y * s), ti = fma(a.x, s, a.y * cm1);
+    return C{a.x + tr, a.y + ti};
+}
 
 __device__ __forceinline__ float recip(float m) { return __builtin_amdgcn_rcpf(m); }
 __device__ __forceinline__ double recip(double m) { return 1.0 / m; }
@@ -137,14 +154,33 @@ __device__ __forceinline__ C poisson_factor(const MulArgs &a, int m, C x)
     return C{x.x * f, x.y * f};
 }
 
-// drift factor: table entry of the line position times the column factor
-// (both FP64), rounded once to the state precision
-template <typename C>
-__device__ __forceinline__ C drift_factor(const double2 &tp, const double2 &cf, C x)
+// drift factor e^{i theta}/S: the table entry of the line position times the
+// column factor, both FP64.  Applied as (x + x (e^{i theta} - 1)) / S with
+// e^{i theta} - 1 formed in FP64, so the FP32 result rounds once, unbiased
+// (S is a power of two).
+__device__ __forceinline__ float2 drift_factor(const double2 &tp, const double2 &cf, double inv_s, float2 x)
 {
     const double2 f = zmul(tp, cf);
-    using Rt = decltype(x.x);
-    return cmul(x, C{(Rt)f.x, (Rt)f.y});
+    const float fr = (float)(f.x - 1.0), fi = (float)f.y;
+    const float tr = fmaf(x.x, fr, -x.y * fi), ti = fmaf(x.x, fi, x.y * fr);
+    const float s = (float)inv_s;
+    return float2{(x.x + tr) * s, (x.y + ti) * s};
+}
+__device__ __forceinline__ double2 drift_factor(const double2 &tp, const double2 &cf, double inv_s, double2 x)
+{
+    const double2 f = zmul(tp, cf);
+    return double2{(x.x * f.x - x.y * f.y) * inv_s, (x.x * f.y + x.y * f.x) * inv_s};
+}
+
+// Work-group dispatch is round-robin over the 8 XCDs (each with its own
+// L2): block b runs on XCD b % 8.  Renumber so that the blocks of one XCD
+// take one contiguous run of tiles; neighbouring column tiles then share
+// their L2 (a 128 B line split across tiles is fetched once).
+__device__ __forceinline__ long long xcd_block()
+{
+    const long long nb = gridDim.x, b = blockIdx.x;
+    if (nb % 8 != 0) return b;
+    return (b % 8) * (nb / 8) + b / 8;
 }
 
 template <int N, typename C>
@@ -214,8 +250,9 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int c = threadIdx.x % TC, t = threadIdx.x / TC;
-    const long long o = blockIdx.x / ntiles;
-    const long long col = (blockIdx.x % ntiles) * TC + c;
+    const long long b = xcd_block();
+    const long long o = b / ntiles;
+    const long long col = (b % ntiles) * TC + c;
     const bool ok = col < inner;
     C *base = data + o * (long long)N * inner + col;
     C v[P::E];
@@ -235,10 +272,8 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
             double2 cf;
             if (mul.geo == GEO_PSI3) cf = zmul(T[cc / N], T[cc % N]);
             else cf = T[cc];
-            cf.x *= ma.inv_s;
-            cf.y *= ma.inv_s;
 #pragma unroll
-            for (int q = 0; q < P::E; ++q) v[q] = drift_factor(T[t + q * P::TPL], cf, v[q]);
+            for (int q = 0; q < P::E; ++q) v[q] = drift_factor(T[t + q * P::TPL], cf, ma.inv_s, v[q]);
         } else {  // GEO_VT3 Poisson
             const int a = foldi(cc, N);
             const int mc = a * a + (int)(o * o);
@@ -455,10 +490,10 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 R sa, ca, sb, cb;
-                sincos_kick(-dta * va[q], &sa, &ca);
-                sincos_kick(-dtb * vb[q], &sb, &cb);
-                a[q] = C{a[q].x * ca - a[q].y * sa, a[q].x * sa + a[q].y * ca};
-                b[q] = C{b[q].x * cb - b[q].y * sb, b[q].x * sb + b[q].y * cb};
+                sincosm1_kick(-dta * va[q], &sa, &ca);
+                sincosm1_kick(-dtb * vb[q], &sb, &cb);
+                a[q] = rotate(a[q], sa, ca);
+                b[q] = rotate(b[q], sb, cb);
             }
         }
         if (flags & C2R_FWD) {

@@ -80,6 +80,10 @@ struct qx_ctx {
     double vmax = 0.0;
     double2 *ptab = nullptr;   // drift phase table (N entries)
     double ptab_dt = std::nan("");
+    // captured steady-state steps for small lattices (launch-bound)
+    hipGraphExec_t gexec = nullptr;
+    double g_ds = std::nan("");
+    qx_physics g_ph{0, 0, -1, -1};
     std::unique_ptr<Engine> eng;
 };
 
@@ -498,6 +502,7 @@ qx_status qx_destroy(qx_ctx *c)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_partials) (void)hipHostFree(c->h_partials);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return QX_OK;
@@ -706,6 +711,44 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
 
 // ---- step compositions --------------------------------------------------------
 
+// n steady-state fused steps.  Lattices of at most 2^20 sites are
+// launch-bound, so their steps are captured once (GRAPH_STEPS per graph,
+// keyed by ds and the physics constants) and replayed.
+static constexpr int GRAPH_STEPS = 8;
+static constexpr long long GRAPH_MAX_SITES = 1ll << 20;
+
+static bool same_physics(const qx_physics &a, const qx_physics &b)
+{
+    return a.pref == b.pref && a.rho_bar == b.rho_bar && a.k0_mode == b.k0_mode &&
+           a.rho_bar_mode == b.rho_bar_mode;
+}
+
+static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, const MulArgs &pm)
+{
+    const int np = step_pass_count(c);
+    int done = 0;
+    if (c->S <= GRAPH_MAX_SITES && n >= GRAPH_STEPS) {
+        if (!c->gexec || !(c->g_ds == ds) || !same_physics(c->g_ph, c->ph)) {
+            if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < GRAPH_STEPS; ++i)
+                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
+            HIPCHK(hipStreamEndCapture(c->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(e);
+            c->g_ds = ds;
+            c->g_ph = c->ph;
+        }
+        for (; done + GRAPH_STEPS <= n; done += GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    }
+    for (; done < n; ++done)
+        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
+    return QX_OK;
+}
+
 static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half, int last_half, bool wait)
 {
     CHECK_CTX(c);
@@ -720,14 +763,11 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
     MulArgs dm;
     TRY(drift_args(c, ds, &dm));
     const int np = step_pass_count(c);
-    for (int i = 0; i < nsteps; ++i) {
-        // the opening kick reads the stored V; later kicks the potential's spectrum
-        if (i == 0)
-            c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->rows, k_first, nullptr, C2R_KICK | C2R_FWD, nullptr, c->stream);
-        else
-            step_pass(c, 0, ds, c->psi, c->vt, dm, pm);
-        for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
-    }
+    // the opening kick reads the stored V; later kicks the potential's spectrum
+    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->rows, k_first, nullptr, C2R_KICK | C2R_FWD, nullptr,
+                     c->stream);
+    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
+    TRY(steady_steps(c, nsteps - 1, ds, dm, pm));
     // last step's c2r: materialise V and max|V|, closing half kick
     if (wait) TRY(reset_vmax(c));
     c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.5 * ds, nullptr,

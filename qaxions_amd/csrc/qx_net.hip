@@ -93,7 +93,7 @@ __global__ void pack_kernel(const float *__restrict__ theta, bf16_t *__restrict_
 // vector convolution: one thread per pixel, all CO outputs
 //   IN = float (first layer, psi) or bf16 (hidden activations)
 //   MODE 0: forward, z = conv + b, a = gelu(z) -> bf16 (hidden layers)
-//   MODE 1: forward last layer: out = x + conv + b written as float [P][2]
+//   MODE 1: forward last layer: out = x + oscale (conv + b), float [P][2]
 //   MODE 2: input gradient of a hidden layer: d_in[p][c] = conv_T(dz) then
 //           multiplied by gelu'(z_prev[p][c]) (the producing layer's pre-act)
 //   MODE 3: input gradient without the GELU factor (gradient w.r.t. psi)
@@ -102,7 +102,7 @@ template <int CI, int CO, int MODE, typename IN>
 __global__ void __launch_bounds__(256)
 conv_vec(const IN *__restrict__ in, int H, int W, const bf16_t *__restrict__ wk,
          const float *__restrict__ bias, float *__restrict__ z_out, void *__restrict__ out,
-         const float *__restrict__ x_res, const float *__restrict__ zprev)
+         const float *__restrict__ x_res, const float *__restrict__ zprev, float oscale = 1.0f)
 {
     __shared__ float ws[9 * CI * CO];
     for (int i = threadIdx.x; i < 9 * CI * CO; i += blockDim.x) ws[i] = bf2f(wk[i]);
@@ -142,7 +142,7 @@ conv_vec(const IN *__restrict__ in, int H, int W, const bf16_t *__restrict__ wk,
     } else if (MODE == 1) {
         float *y = (float *)out;
 #pragma unroll
-        for (int o = 0; o < CO; ++o) y[p * CO + o] = x_res[p * CO + o] + acc[o];
+        for (int o = 0; o < CO; ++o) y[p * CO + o] = fmaf(oscale, acc[o], x_res[p * CO + o]);
     } else if (MODE == 2) {
         float *d = (float *)out;
 #pragma unroll
@@ -314,14 +314,15 @@ __global__ void wgrad_reduce(const float *__restrict__ partial, int nblocks, int
     }
 }
 
-// residual r = y - target, d = 2 r / n, per-block partial of sum r^2
+// residual r = y - target; d = oscale 2 r / n is the gradient with respect
+// to the last layer's output; per-block partial of sum r^2
 __global__ void __launch_bounds__(256)
-loss_kernel(const float *__restrict__ y, const float *__restrict__ t, long long n, float *__restrict__ d,
-            double *__restrict__ partial)
+loss_kernel(const float *__restrict__ y, const float *__restrict__ t, long long n, float oscale,
+            float *__restrict__ d, double *__restrict__ partial)
 {
     __shared__ double sh[4];
     double s = 0.0;
-    const float sc = (float)(2.0 / (double)n);
+    const float sc = (float)(2.0 * (double)oscale / (double)n);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const float r = y[i] - t[i];
@@ -371,6 +372,7 @@ struct qx_net {
     double *lpart = nullptr, *h_lpart = nullptr;
     float *xbuf = nullptr, *tbuf = nullptr;  // host-array staging
     int adam_t = 0;
+    float out_scale = 1.0f;  // y = x + out_scale * net(x)
     bool packed = false;
     int wchunk = 512, wblocks = 0;
     void *save_psi = nullptr, *save_V = nullptr;  // fallback copies (lazy)
@@ -424,7 +426,7 @@ qx_status forward(qx_net *n, const float *x, float *y, bool keep, hipStream_t s 
         }
     }
     conv_vec<32, 2, 1, bf16_t><<<gp, 256, 0, s>>>(n->a[2], n->H, n->W, n->wf + wofs(3), bias(n, 3), nullptr,
-                                                  y, x, nullptr);
+                                                  y, x, nullptr, n->out_scale);
     NCHK(hipGetLastError());
     return QX_OK;
 }
@@ -443,7 +445,7 @@ qx_status loss_grad(qx_net *n, const float *x, const float *t, double *loss)
     hipStream_t s = n->stream;
     const long long ne = n->P * 2;
     const unsigned gl = grid1(ne);
-    loss_kernel<<<gl, 256, 0, s>>>(n->y, t, ne, n->d0, n->lpart);
+    loss_kernel<<<gl, 256, 0, s>>>(n->y, t, ne, n->out_scale, n->d0, n->lpart);
     // layer 3 (32 -> 2, no activation): dz3 = d0 [P][2]
     const unsigned gp = (unsigned)((n->P + 255) / 256);
     const int wb = n->wblocks;
@@ -622,6 +624,14 @@ qx_status qx_net_train_step(qx_net *n, const void *x_dev, const void *t_dev, flo
 }
 
 void *qx_net_stream(qx_net *n) { return n ? (void *)n->stream : nullptr; }
+
+qx_status qx_net_set_output_scale(qx_net *n, float s)
+{
+    if (!n || !(s > 0.0f)) return nfail(QX_EINVAL, "output scale must be positive");
+    n->out_scale = s;
+    return QX_OK;
+}
+float qx_net_output_scale(const qx_net *n) { return n ? n->out_scale : 0.0f; }
 
 static qx_status check_ctx(qx_net *n, qx_ctx *ctx, float **psi, float **V)
 {

@@ -41,6 +41,7 @@ def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, 
     total = len(seeds) * steps_per_seed
     losses = []
     it = 0
+    scale_set = net.output_scale != 1.0
     for seed in seeds:
         traj.ic_grf(int(seed))
         traj.update_potential()
@@ -49,6 +50,11 @@ def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, 
             coarse.leapfrog(k * dt, 1)
             fine.copy_from(traj)
             fine.leapfrog(dt, k)
+            if not scale_set:
+                # the network predicts the correction in units of its typical size
+                d = fine.psi - coarse.psi
+                net.output_scale = float(np.sqrt(np.mean(np.abs(d) ** 2) / 2.0))
+                scale_set = True
             lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(total - 1, 1))
             losses.append(net.train_step_lattices(coarse, fine, lr_it))
             traj.copy_from(fine)

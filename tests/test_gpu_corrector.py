@@ -36,8 +36,10 @@ def test_gradient_matches_oracle(qx, H, W):
     x = rng.standard_normal((H, W, 2)).astype(np.float32)
     t = (x + 0.1 * rng.standard_normal((H, W, 2))).astype(np.float32)
     net = qx.Corrector(H, W, params=theta)
+    net.output_scale = 0.25
     loss, g = net.loss_grad(x, t)
-    lo, go = cn.loss_and_grad(theta.astype(np.float64), x.astype(np.float64), t.astype(np.float64), bf16=True)
+    lo, go = cn.loss_and_grad(theta.astype(np.float64), x.astype(np.float64), t.astype(np.float64), bf16=True,
+                              scale=0.25)
     assert loss == pytest.approx(lo, rel=1e-3)
     assert rel(g, go) < 2e-2
     # every layer's weights and biases individually
