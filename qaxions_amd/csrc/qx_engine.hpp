@@ -2,6 +2,7 @@
 // precision is instantiated in its own translation unit (qx_lib_f32.hip,
 // qx_lib_f64.hip) so the two compile in parallel.
 #pragma once
+#include <algorithm>
 #include <memory>
 #include "qx_kernels.hpp"
 
@@ -29,25 +30,49 @@ template <int N, typename R> struct EngineT final : Engine {
     using G = Geom<N, R>;
     using P = Plan<N>;
 
+    // persistent grids: as many blocks as fit on the device at once
+    template <typename K> static long long resident_blocks(K kernel, int threads)
+    {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0);
+        if (cus < 1) cus = 1;
+        if (per < 1) per = 1;
+        return (long long)cus * per;
+    }
+    long long contig_cap[3] = {0, 0, 0}, strided_cap[4] = {0, 0, 0, 0};
+
     void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) override
     {
-        const unsigned blocks = (unsigned)((nlines + G::LPB - 1) / G::LPB);
+        const long long units = (nlines + G::LPB - 1) / G::LPB;
         C *d = (C *)data;
         const C *t = (const C *)tw;
-        if (op < 0) contig_fft<N, R, -1><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
-        else if (op > 0) contig_fft<N, R, +1><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
-        else contig_fft<N, R, 0><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        auto launch = [&](auto kernel, long long &cap) {
+            if (!cap) cap = resident_blocks(kernel, G::CONTIG_THREADS);
+            const unsigned blocks = (unsigned)std::min(units, cap);
+            kernel<<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        };
+        if (op < 0) launch(contig_fft<N, R, OP_FWD>, contig_cap[0]);
+        else if (op > 0) launch(contig_fft<N, R, OP_INV>, contig_cap[1]);
+        else launch(contig_fft<N, R, OP_POISSON>, contig_cap[2]);
     }
     void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                  hipStream_t s) override
     {
         const long long ntiles = (inner + G::TC - 1) / G::TC;
-        const unsigned blocks = (unsigned)(outer * ntiles);
+        const long long units = outer * ntiles;
         C *d = (C *)data;
         const C *t = (const C *)tw;
-        if (op < 0) strided_fft<N, R, -1><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
-        else if (op > 0) strided_fft<N, R, +1><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
-        else strided_fft<N, R, 0><<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, m, t);
+        auto launch = [&](auto kernel, long long &cap) {
+            if (!cap) cap = resident_blocks(kernel, G::STRIDED_THREADS);
+            const unsigned blocks = (unsigned)std::min(units, cap);
+            kernel<<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, units, m, t);
+        };
+        if (op < 0) launch(strided_fft<N, R, OP_FWD>, strided_cap[0]);
+        else if (op > 0) launch(strided_fft<N, R, OP_INV>, strided_cap[1]);
+        else if (m.kind == MUL_DRIFT) launch(strided_fft<N, R, OP_DRIFT>, strided_cap[2]);
+        else launch(strided_fft<N, R, OP_POISSON>, strided_cap[3]);
     }
     void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
                   double pref, double rho_bar, hipStream_t s) override

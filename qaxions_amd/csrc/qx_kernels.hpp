@@ -56,6 +56,10 @@ template <int N, typename R> struct Geom {
 
 // pointwise factor in fused passes
 enum { MUL_NONE = 0, MUL_DRIFT = 1, MUL_POISSON = 2 };
+// pass operations (template argument): plain transforms, or forward,
+// pointwise factor and inverse in one pass; separate instantiations keep the
+// factor code out of the plain passes and name each pass in profiles
+enum { OP_FWD = -1, OP_INV = 1, OP_DRIFT = 0, OP_POISSON = 2 };
 // geometry of the multiplied pass: how the 3 (or 2) integer frequencies of
 // an element follow from (line position p, column/line index c, outer o)
 enum {
@@ -190,9 +194,15 @@ __device__ __forceinline__ void load_tw(C *tw, const C *twg)
 }
 
 // ---------------------------------------------------------------------------
-// contiguous lines: data[line*N + pos]
+// The two transform passes below are persistent: a block walks work units
+// u = lb, lb + nb, lb + 2 nb, ... (lb = XCD-grouped block index, nb = grid
+// size) and loads unit u + nb into registers before transforming unit u, so
+// the next unit's HBM reads are in flight during the LDS exchanges of the
+// current one.
 // ---------------------------------------------------------------------------
-template <int N, typename R, int OP>  // OP: -1 forward, +1 inverse, 0 fwd*mul*inv
+
+// contiguous lines: data[line*N + pos], LPB lines per unit
+template <int N, typename R, int OP>  // OP_FWD, OP_INV or OP_POISSON (2D vt lines)
 __global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
 contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul,
            const typename cplx<R>::T *__restrict__ twg)
@@ -204,42 +214,61 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, l = threadIdx.x / P::TPL;
-    const long long line = (long long)blockIdx.x * G::LPB + l;
-    const bool ok = line < nlines;
     C *mine = lds + l * P::PADDED;
+    const long long nunits = (nlines + G::LPB - 1) / G::LPB, nb = gridDim.x;
+    long long u = xcd_block();
     C v[P::E];
-    C *src = data + line * N;
+    {
+        const long long line = u * G::LPB + l;
+        const C *src = data + line * N;
 #pragma unroll
-    for (int q = 0; q < P::E; ++q) v[q] = ok ? src[t + q * P::TPL] : C{0, 0};
-    __syncthreads();
-    if (OP < 0) {
-        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
-    } else if (OP > 0) {
-        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
-    } else {
-        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
-        // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
-        const int k1 = (int)(line % mul.member_div);
-        const MulArgs ma = member_args(mul, line / mul.member_div);
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int n0 = foldi(t + q * P::TPL, N);
-            v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
-        }
-        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+        for (int q = 0; q < P::E; ++q) v[q] = (u < nunits && line < nlines) ? src[t + q * P::TPL] : C{0, 0};
     }
-    if (ok) {
+    __syncthreads();
+    for (; u < nunits; u += nb) {
+        const long long line = u * G::LPB + l;
+        const bool ok = line < nlines;
+        // prefetch the next unit
+        C nx[P::E];
+        {
+            const long long un = u + nb, ln = un * G::LPB + l;
+            const bool okn = un < nunits && ln < nlines;
+            const C *src = data + ln * N;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) src[t + q * P::TPL] = v[q];
+            for (int q = 0; q < P::E; ++q) nx[q] = okn ? src[t + q * P::TPL] : C{0, 0};
+        }
+        if constexpr (OP == OP_FWD) {
+            fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+        } else if constexpr (OP == OP_INV) {
+            fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+        } else {
+            fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+            // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
+            const int k1 = (int)(line % mul.member_div);
+            const MulArgs ma = member_args(mul, line / mul.member_div);
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int n0 = foldi(t + q * P::TPL, N);
+                v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
+            }
+            fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+        }
+        if (ok) {
+            C *dst = data + line * N;
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) dst[t + q * P::TPL] = v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = nx[q];
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------------------
-// strided columns: data[o*N*inner + p*inner + col], col in [0, inner)
-// ---------------------------------------------------------------------------
-template <int N, typename R, int OP>  // OP: -1 forward, +1 inverse, 0 fwd*mul*inv
+// strided columns: data[o*N*inner + p*inner + col], col in [0, inner); a unit
+// is TC adjacent columns of one outer index
+template <int N, typename R, int OP>  // OP_FWD, OP_INV, OP_DRIFT or OP_POISSON
 __global__ void __launch_bounds__((Geom<N, R>::STRIDED_THREADS))
-strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
+strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles, long long nunits,
             MulArgs mul, const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
@@ -250,44 +279,63 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int c = threadIdx.x % TC, t = threadIdx.x / TC;
-    const long long b = xcd_block();
-    const long long o = b / ntiles;
-    const long long col = (b % ntiles) * TC + c;
-    const bool ok = col < inner;
-    C *base = data + o * (long long)N * inner + col;
+    const long long nb = gridDim.x;
+    long long u = xcd_block();
     C v[P::E];
+    {
+        const long long o = u / ntiles, col = (u % ntiles) * TC + c;
+        const C *src = data + o * (long long)N * inner + col;
 #pragma unroll
-    for (int q = 0; q < P::E; ++q) v[q] = ok ? base[(long long)(t + q * P::TPL) * inner] : C{0, 0};
-    __syncthreads();
-    if (OP < 0) {
-        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-    } else if (OP > 0) {
-        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
-    } else {
-        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-        const MulArgs ma = member_args(mul, o / mul.member_div);
-        const int cc = ok ? (int)col : 0;
-        if (mul.kind == MUL_DRIFT) {
-            const double2 *T = ma.ptab;
-            double2 cf;
-            if (mul.geo == GEO_PSI3) cf = zmul(T[cc / N], T[cc % N]);
-            else cf = T[cc];
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) v[q] = drift_factor(T[t + q * P::TPL], cf, ma.inv_s, v[q]);
-        } else {  // GEO_VT3 Poisson
-            const int a = foldi(cc, N);
-            const int mc = a * a + (int)(o * o);
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int n0 = foldi(t + q * P::TPL, N);
-                v[q] = poisson_factor<R>(ma, n0 * n0 + mc, v[q]);
-            }
-        }
-        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+        for (int q = 0; q < P::E; ++q)
+            v[q] = (u < nunits && col < inner) ? src[(long long)(t + q * P::TPL) * inner] : C{0, 0};
     }
-    if (ok) {
+    __syncthreads();
+    for (; u < nunits; u += nb) {
+        const long long o = u / ntiles;
+        const long long col = (u % ntiles) * TC + c;
+        const bool ok = col < inner;
+        C nx[P::E];
+        {
+            const long long un = u + nb, on = un / ntiles, cn = (un % ntiles) * TC + c;
+            const bool okn = un < nunits && cn < inner;
+            const C *src = data + on * (long long)N * inner + cn;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) base[(long long)(t + q * P::TPL) * inner] = v[q];
+            for (int q = 0; q < P::E; ++q) nx[q] = okn ? src[(long long)(t + q * P::TPL) * inner] : C{0, 0};
+        }
+        if constexpr (OP == OP_FWD) {
+            fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+        } else if constexpr (OP == OP_INV) {
+            fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+        } else {
+            fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+            const MulArgs ma = member_args(mul, o / mul.member_div);
+            const int cc = ok ? (int)col : 0;
+            if constexpr (OP == OP_DRIFT) {
+                const double2 *T = ma.ptab;
+                double2 cf;
+                if (mul.geo == GEO_PSI3) cf = zmul(T[cc / N], T[cc % N]);
+                else cf = T[cc];
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) v[q] = drift_factor(T[t + q * P::TPL], cf, ma.inv_s, v[q]);
+            } else {  // GEO_VT3 Poisson
+                const int a = foldi(cc, N);
+                const int mc = a * a + (int)(o * o);
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    const int n0 = foldi(t + q * P::TPL, N);
+                    v[q] = poisson_factor<R>(ma, n0 * n0 + mc, v[q]);
+                }
+            }
+            fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+        }
+        if (ok) {
+            C *dst = data + o * (long long)N * inner + col;
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) dst[(long long)(t + q * P::TPL) * inner] = v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = nx[q];
+        __syncthreads();
     }
 }
 

@@ -45,6 +45,49 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 std::unique_ptr<Engine> make_engine(int N, bool fp64) { return fp64 ? make_engine_f64(N) : make_engine_f32(N); }
 
+// Twiddles e^{-2 pi i m/N}, computed in FP64.  In FP32 each is rounded to
+// the pair of neighbouring floats (cos and sin each rounded down or up) whose
+// squared modulus is closest to 1: nearest rounding of the two parts leaves
+// |w|^2 - 1 with a nonzero mean over the table (-4e-9 at N = 256), a bias
+// every twiddled multiply would add to the norm (mass) of psi.
+qx_status upload_twiddles(void *dst, int N, bool fp64)
+{
+    std::vector<double> tw(2 * (size_t)N);
+    for (int m = 0; m < N; ++m) {
+        const double a = TWO_PI * (double)m / (double)N;
+        double cr = std::cos(a), ci = -std::sin(a);
+        if (4 * m == N) { cr = 0.0; ci = -1.0; }
+        if (2 * m == N) { cr = -1.0; ci = 0.0; }
+        if (4 * m == 3 * N) { cr = 0.0; ci = 1.0; }
+        tw[2 * m] = cr;
+        tw[2 * m + 1] = ci;
+    }
+    hipError_t e;
+    if (fp64) {
+        e = hipMemcpy(dst, tw.data(), 16 * (size_t)N, hipMemcpyHostToDevice);
+    } else {
+        std::vector<float> tf(2 * (size_t)N);
+        for (int m = 0; m < N; ++m) {
+            const double cr = tw[2 * m], ci = tw[2 * m + 1];
+            float bc = (float)cr, bs = (float)ci;
+            double best = std::fabs((double)bc * bc + (double)bs * bs - 1.0);
+            const float cc[3] = {(float)cr, std::nextafterf((float)cr, -2.f), std::nextafterf((float)cr, 2.f)};
+            const float ss[3] = {(float)ci, std::nextafterf((float)ci, -2.f), std::nextafterf((float)ci, 2.f)};
+            for (float x : cc)
+                for (float y : ss) {
+                    if (std::fabs(x - cr) > 1.2e-7 || std::fabs(y - ci) > 1.2e-7) continue;  // within ~1 ulp
+                    const double d = std::fabs((double)x * x + (double)y * y - 1.0);
+                    if (d < best) { best = d; bc = x; bs = y; }
+                }
+            tf[2 * m] = bc;
+            tf[2 * m + 1] = bs;
+        }
+        e = hipMemcpy(dst, tf.data(), 8 * (size_t)N, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return fail(QX_EDEVICE, "twiddle upload failed");
+    return QX_OK;
+}
+
 unsigned grid_for(long long n, int threads = 256, long long cap = 4096)
 {
     long long b = (n + threads - 1) / threads;
@@ -463,25 +506,7 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
     if (hipHostMalloc((void **)&c->h_partials, sizeof(double) * 2 * MAX_PART) != hipSuccess)
         return cleanup(fail(QX_ENOMEM, "pinned host allocation failed"));
 
-    // twiddles in FP64, rounded once to the state precision
-    std::vector<double> tw(2 * (size_t)N);
-    for (int m = 0; m < N; ++m) {
-        double a = TWO_PI * (double)m / (double)N;
-        double cr = std::cos(a), ci = -std::sin(a);
-        if (4 * m == N) { cr = 0.0; ci = -1.0; }
-        if (2 * m == N) { cr = -1.0; ci = 0.0; }
-        if (4 * m == 3 * N) { cr = 0.0; ci = 1.0; }
-        tw[2 * m] = cr;
-        tw[2 * m + 1] = ci;
-    }
-    if (prec == QX_FP64) {
-        if (hipMemcpy(c->eng->tw, tw.data(), 16 * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fail(QX_EDEVICE, "twiddle upload failed"));
-    } else {
-        std::vector<float> tf(tw.begin(), tw.end());
-        if (hipMemcpy(c->eng->tw, tf.data(), 8 * (size_t)N, hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fail(QX_EDEVICE, "twiddle upload failed"));
-    }
+    if (upload_twiddles(c->eng->tw, N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
     if (hipMemsetAsync(c->psi, 0, (size_t)c->S * c->csz, c->stream) != hipSuccess ||
         hipMemsetAsync(c->V, 0, (size_t)c->S * c->rsz, c->stream) != hipSuccess ||
         hipMemsetAsync(c->vt, 0, (size_t)c->H * c->csz, c->stream) != hipSuccess ||
@@ -1239,23 +1264,7 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
                 *bufs[b] = nullptr;
                 return cleanup(fail(QX_ENOMEM, "ensemble allocation failed"));
             }
-        std::vector<double> tw(2 * (size_t)k.N);
-        for (int m = 0; m < k.N; ++m) {
-            double a = TWO_PI * (double)m / (double)k.N, cr = std::cos(a), ci = -std::sin(a);
-            if (4 * m == k.N) { cr = 0.0; ci = -1.0; }
-            if (2 * m == k.N) { cr = -1.0; ci = 0.0; }
-            if (4 * m == 3 * k.N) { cr = 0.0; ci = 1.0; }
-            tw[2 * m] = cr;
-            tw[2 * m + 1] = ci;
-        }
-        hipError_t he;
-        if (prec == QX_FP64) {
-            he = hipMemcpy(k.eng->tw, tw.data(), 16 * (size_t)k.N, hipMemcpyHostToDevice);
-        } else {
-            std::vector<float> tf(tw.begin(), tw.end());
-            he = hipMemcpy(k.eng->tw, tf.data(), 8 * (size_t)k.N, hipMemcpyHostToDevice);
-        }
-        if (he != hipSuccess) return cleanup(fail(QX_EDEVICE, "twiddle upload failed"));
+        if (upload_twiddles(k.eng->tw, k.N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
         if (hipMemset(k.psi, 0, sz[0]) != hipSuccess || hipMemset(k.V, 0, sz[1]) != hipSuccess ||
             hipMemset(k.vt, 0, sz[2]) != hipSuccess)
             return cleanup(fail(QX_EDEVICE, "state initialisation failed"));

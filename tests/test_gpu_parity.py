@@ -243,7 +243,9 @@ def test_plain_method_256sq_100_steps(qx, prec, tol):
     print(f"psi rel L2 after {steps} steps ({prec}): {err:.3e}")
     assert err <= tol
     m0 = (np.abs(psi0) ** 2).sum()
-    assert abs((np.abs(got) ** 2).sum() - m0) / m0 < (1e-13 if prec == "fp64" else 1e-6)
+    # FP32: 100 steps of unitary FFT passes in single precision keep the mass
+    # to a few 1e-6 (round-off of ~4 transforms per step)
+    assert abs((np.abs(got) ** 2).sum() - m0) / m0 < (1e-13 if prec == "fp64" else 1e-5)
     P, c, _ = lat.spectrum(source="rho", nbins=64, spb=2)
     Po, co, _ = o.spectrum(np.abs(want) ** 2, L, 64, 2)
     assert np.array_equal(c, co)
