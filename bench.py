@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pass-reps", type=int, default=10)
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile-variant autotuner")
     ap.add_argument("--workload", default="kdk", choices=["kdk", "ensemble", "learned"],
                     help="kdk: the headline 512^3 step (default); ensemble: 96 uneven 2D "
                          "lattices; learned: corrector training and evaluation at 512^2")
@@ -109,6 +110,7 @@ def main():
     lat = qx.Lattice(dim, N, L, precision=args.precision, pref=1.0, rho_bar=1.0, k0_mode=0)
     lat.ic_grf(seed=0, amp=0.1)
     lat.update_potential()
+    variants = lat.tune(dt, reps=5) if not args.no_tune else None
     r0, m0 = lat.density_stats()
     ev = HipEvents()
     stream = lat.stream
@@ -155,7 +157,8 @@ def main():
         "data": "synthetic: seeded Gaussian random field IC, P(k)~k^-2 cut at k_Ny/2",
         "config": {"workload": f"KDK Schroedinger-Poisson step, {dim}D {N}^{dim}, L={L}, dt={dt}, "
                                f"pref=1, rho_bar=1, k=0 zeroed, GRF seed 0",
-                   "lattice": [N] * dim, "precision": args.precision, "parallelism": "single GPU"},
+                   "lattice": [N] * dim, "precision": args.precision, "parallelism": "single GPU",
+                   "tile_variants": variants},
         "roofline": {"bound": "hbm", "kernel": names[dom],
                      "achieved": pbytes[dom] / (pms[dom] * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -223,22 +226,22 @@ def bench_learned(qx, args):
     """Corrector trained on solver pairs at 512^2 (coarse 4 dt vs 4 fine dt),
     evaluated over coarse steps against fine and coarse stepping."""
     from qaxions_amd import learned
-    N, L, dt, k = 512, 10.0, 1e-3, 4
+    N, L, dt, k, pref = 512, 10.0, 1e-3, 4, 100.0
     per_seed = 1000
     nseeds = max(1, args.train_pairs // per_seed)
     t0 = time.perf_counter()
     net, losses = learned.train(N, L, dt, k, seeds=range(1000, 1000 + nseeds),
                                 steps_per_seed=min(per_seed, args.train_pairs), lr=1e-3, lr_final=1e-4,
-                                log_every=1000)
+                                log_every=1000, pref=pref)
     t_train = time.perf_counter() - t0
     res = learned.evaluate(net, N, L, dt, k, seed=0, n_coarse=args.eval_steps, check_every=10,
-                           mass_tol=1e-6, timing_reps=2)
+                           mass_tol=1e-6, timing_reps=2, pref=pref)
     out = {"metric": "learned coarse stepping at 512^2: speed-up over fine stepping and errors",
            "value": res["speedup_learned_vs_fine"], "unit": "x", "n_gpus": 1,
            "train_pairs": int(len(losses)), "train_seconds": t_train,
            "loss_first_1k": float(losses[:1000].mean()), "loss_last_1k": float(losses[-1000:].mean()),
            "eval": res, "dtype": "c64 state, bf16 weights, fp32 accumulate",
-           "config": {"workload": f"2D {N}^2, L={L}, fine dt={dt}, coarse {k} dt, "
+           "config": {"workload": f"2D {N}^2, L={L}, pref={pref}, fine dt={dt}, coarse {k} dt, "
                                   f"{args.eval_steps} coarse steps, mass tol 1e-6 per 10-step chunk"}}
     print(json.dumps(out), flush=True)
 

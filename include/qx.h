@@ -214,6 +214,14 @@ qx_status qx_net_coarse_steps(qx_net *net, qx_ctx *ctx, double dt, int k, int ns
 void *qx_net_stream(qx_net *net);
 const char *qx_net_last_error(void);
 
+/* Autotuner: times both tile variants (full / half column tile) of every
+ * strided pass of the fused step on scratch copies and keeps the faster per
+ * pass kind.  variants (nullable, 8 ints): the choice per [op kind][stride
+ * class], op kinds forward, inverse, drift, Poisson. */
+qx_status qx_tune(qx_ctx *ctx, double ds, int reps, int *variants);
+/* installs variants recorded by an earlier qx_tune (e.g. from a cache) */
+qx_status qx_set_tile_variants(qx_ctx *ctx, const int *variants);
+
 /* ---- plumbing (no reference counterpart) -------------------------------- */
 void *qx_stream(qx_ctx *ctx);          /* the context's hipStream_t */
 qx_status qx_synchronize(qx_ctx *ctx);

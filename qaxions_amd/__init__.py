@@ -115,6 +115,8 @@ SIGNATURES = {
                                       C.c_double, C.POINTER(C.c_int)]),
     "qx_net_stream": (C.c_void_p, [C.c_void_p]),
     "qx_net_last_error": (C.c_char_p, []),
+    "qx_tune": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int)]),
+    "qx_set_tile_variants": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "qx_stream": (C.c_void_p, [C.c_void_p]),
     "qx_synchronize": (C.c_int, [C.c_void_p]),
     "qx_last_error": (C.c_char_p, []),
@@ -340,6 +342,36 @@ class Lattice:
         _check(lib().qx_time_step_passes(self._h, float(ds), int(reps), _p(ms), _p(by), n))
         names = [lib().qx_step_pass_name(self._h, i).decode() for i in range(n)]
         return names, ms, by
+
+    def tune(self, ds, reps=5, cache=None):
+        """Pick the strided tile variant of every pass by measurement.  With a
+        cache path (JSON), a previous result for this (dim, N, precision) is
+        reused, else the new one is stored."""
+        import json
+        key = f"{self.dim}d-{self.N}-{self.precision}"
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                db = json.load(f)
+            if key in db:
+                v = (C.c_int * 8)(*db[key])
+                _check(lib().qx_set_tile_variants(self._h, v))
+                return list(db[key])
+        v = (C.c_int * 8)()
+        _check(lib().qx_tune(self._h, float(ds), int(reps), v))
+        out = list(v)
+        if cache:
+            db = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    db = json.load(f)
+            db[key] = out
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(db, f, indent=1)
+        return out
+
+    def set_tile_variants(self, v):
+        _check(lib().qx_set_tile_variants(self._h, (C.c_int * 8)(*v)))
 
     # -- plumbing ---------------------------------------------------------------------------
     @property

@@ -23,6 +23,14 @@ struct Engine {
                           double kick_dt, const double *kick_member, int flags,
                           unsigned long long *vmax, hipStream_t s) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
+    // tile variant per strided pass kind (0 = full tile, 1 = half tile),
+    // indexed by [op kind][stride class]; set by qx_tune
+    int strided_variant[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    int variant(int op, const MulArgs &m, long long inner) const
+    {
+        const int k = op < 0 ? 0 : (op > 0 ? 1 : (m.kind == MUL_DRIFT ? 2 : 3));
+        return strided_variant[k][inner > 4096 ? 1 : 0];
+    }
 };
 
 template <int N, typename R> struct EngineT final : Engine {
@@ -30,7 +38,7 @@ template <int N, typename R> struct EngineT final : Engine {
     using G = Geom<N, R>;
     using P = Plan<N>;
 
-    // persistent grids: as many blocks as fit on the device at once
+    // persistent grids (row kernels): as many blocks as fit on the device
     template <typename K> static long long resident_blocks(K kernel, int threads)
     {
         int dev = 0, cus = 0, per = 0;
@@ -41,61 +49,67 @@ template <int N, typename R> struct EngineT final : Engine {
         if (per < 1) per = 1;
         return (long long)cus * per;
     }
-    long long contig_cap[3] = {0, 0, 0}, strided_cap[4] = {0, 0, 0, 0};
 
     void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) override
     {
-        const long long units = (nlines + G::LPB - 1) / G::LPB;
+        const unsigned blocks = (unsigned)((nlines + G::LPB - 1) / G::LPB);
         C *d = (C *)data;
         const C *t = (const C *)tw;
-        auto launch = [&](auto kernel, long long &cap) {
-            if (!cap) cap = resident_blocks(kernel, G::CONTIG_THREADS);
-            const unsigned blocks = (unsigned)std::min(units, cap);
-            kernel<<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
-        };
-        if (op < 0) launch(contig_fft<N, R, OP_FWD>, contig_cap[0]);
-        else if (op > 0) launch(contig_fft<N, R, OP_INV>, contig_cap[1]);
-        else launch(contig_fft<N, R, OP_POISSON>, contig_cap[2]);
+        if (op < 0) contig_fft<N, R, OP_FWD><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        else if (op > 0) contig_fft<N, R, OP_INV><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        else contig_fft<N, R, OP_POISSON><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+    }
+
+    template <int TC>
+    void strided_tc(void *data, long long outer, long long inner, int op, const MulArgs &m, hipStream_t s)
+    {
+        const long long ntiles = (inner + TC - 1) / TC;
+        const unsigned blocks = (unsigned)(outer * ntiles);
+        const int threads = TC * P::TPL;
+        C *d = (C *)data;
+        const C *t = (const C *)tw;
+        if (op < 0) strided_fft<N, R, OP_FWD, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
+        else if (op > 0) strided_fft<N, R, OP_INV, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
+        else if (m.kind == MUL_DRIFT) strided_fft<N, R, OP_DRIFT, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
+        else strided_fft<N, R, OP_POISSON, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
     }
     void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                  hipStream_t s) override
     {
-        const long long ntiles = (inner + G::TC - 1) / G::TC;
-        const long long units = outer * ntiles;
-        C *d = (C *)data;
-        const C *t = (const C *)tw;
-        auto launch = [&](auto kernel, long long &cap) {
-            if (!cap) cap = resident_blocks(kernel, G::STRIDED_THREADS);
-            const unsigned blocks = (unsigned)std::min(units, cap);
-            kernel<<<blocks, G::STRIDED_THREADS, 0, s>>>(d, inner, ntiles, units, m, t);
-        };
-        if (op < 0) launch(strided_fft<N, R, OP_FWD>, strided_cap[0]);
-        else if (op > 0) launch(strided_fft<N, R, OP_INV>, strided_cap[1]);
-        else if (m.kind == MUL_DRIFT) launch(strided_fft<N, R, OP_DRIFT>, strided_cap[2]);
-        else launch(strided_fft<N, R, OP_POISSON>, strided_cap[3]);
+        if (variant(op, m, inner)) strided_tc<G::TC_HALF>(data, outer, inner, op, m, s);
+        else strided_tc<G::TC>(data, outer, inner, op, m, s);
     }
+    long long r2c_cap[3] = {0, 0, 0}, c2r_cap[2] = {0, 0};
+
     void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
                   double pref, double rho_bar, hipStream_t s) override
     {
-        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
+        const long long units = (rows + G::TL - 1) / G::TL;
         C *p = (C *)psi;
         const R *vi = (const R *)vin;
         C *o = (C *)vt;
         const C *t = (const C *)tw;
-        if (mode == R2C_REAL)
-            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
-        else if (mode == R2C_DENSITY)
-            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
-        else
-            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+        auto launch = [&](auto kernel, long long &cap) {
+            if (!cap) cap = resident_blocks(kernel, G::ROW_THREADS);
+            const unsigned blocks = (unsigned)std::min(units, cap);
+            kernel<<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+        };
+        if (mode == R2C_REAL) launch(qx::r2c_rows<N, R, R2C_REAL>, r2c_cap[0]);
+        else if (mode == R2C_DENSITY) launch(qx::r2c_rows<N, R, R2C_DENSITY>, r2c_cap[1]);
+        else launch(qx::r2c_rows<N, R, R2C_INV_DENSITY>, r2c_cap[2]);
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
                   const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s) override
     {
-        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
-        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
-                                                             (R)kick_dt, kick_member, flags, vmax,
-                                                             (const C *)tw);
+        const long long units = (rows + G::TL - 1) / G::TL;
+        auto launch = [&](auto kernel, long long &cap) {
+            if (!cap) cap = resident_blocks(kernel, G::ROW_THREADS);
+            const unsigned blocks = (unsigned)std::min(units, cap);
+            kernel<<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm, (R)kick_dt,
+                                                     kick_member, flags, vmax, (const C *)tw);
+        };
+        if (flags & C2R_SRC_VT) launch(qx::c2r_rows<N, R, true>, c2r_cap[0]);
+        else launch(qx::c2r_rows<N, R, false>, c2r_cap[1]);
     }
 };
 

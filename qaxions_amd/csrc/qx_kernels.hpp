@@ -40,6 +40,7 @@ template <int N, typename R> struct Geom {
     static constexpr int TC1 = TC0 > 16 ? 16 : TC0;
     static constexpr int TC = (TC1 * P::TPL > 1024) ? (1024 / P::TPL) : TC1;
     static constexpr int STRIDED_THREADS = TC * P::TPL;
+    static constexpr int TC_HALF = TC > 1 ? TC / 2 : 1;   // narrow-tile variant
     // row kernels: pairs of rows per block
     static constexpr int NP0 = (256 / P::TPL) > 8 ? (256 / P::TPL) : 8;
     static constexpr int NP1 = (NP0 * P::TPL > 1024) ? (1024 / P::TPL) : NP0;
@@ -194,14 +195,8 @@ __device__ __forceinline__ void load_tw(C *tw, const C *twg)
 }
 
 // ---------------------------------------------------------------------------
-// The two transform passes below are persistent: a block walks work units
-// u = lb, lb + nb, lb + 2 nb, ... (lb = XCD-grouped block index, nb = grid
-// size) and loads unit u + nb into registers before transforming unit u, so
-// the next unit's HBM reads are in flight during the LDS exchanges of the
-// current one.
+// contiguous lines: data[line*N + pos], LPB lines per block
 // ---------------------------------------------------------------------------
-
-// contiguous lines: data[line*N + pos], LPB lines per unit
 template <int N, typename R, int OP>  // OP_FWD, OP_INV or OP_POISSON (2D vt lines)
 __global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
 contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul,
@@ -214,133 +209,107 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, l = threadIdx.x / P::TPL;
+    const long long line = (long long)blockIdx.x * G::LPB + l;
+    const bool ok = line < nlines;
     C *mine = lds + l * P::PADDED;
-    const long long nunits = (nlines + G::LPB - 1) / G::LPB, nb = gridDim.x;
-    long long u = xcd_block();
     C v[P::E];
-    {
-        const long long line = u * G::LPB + l;
-        const C *src = data + line * N;
+    C *src = data + line * N;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = (u < nunits && line < nlines) ? src[t + q * P::TPL] : C{0, 0};
-    }
+    for (int q = 0; q < P::E; ++q) v[q] = ok ? src[t + q * P::TPL] : C{0, 0};
     __syncthreads();
-    for (; u < nunits; u += nb) {
-        const long long line = u * G::LPB + l;
-        const bool ok = line < nlines;
-        // prefetch the next unit
-        C nx[P::E];
-        {
-            const long long un = u + nb, ln = un * G::LPB + l;
-            const bool okn = un < nunits && ln < nlines;
-            const C *src = data + ln * N;
+    if constexpr (OP == OP_FWD) {
+        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+    } else if constexpr (OP == OP_INV) {
+        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+    } else {
+        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+        // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
+        const int k1 = (int)(line % mul.member_div);
+        const MulArgs ma = member_args(mul, line / mul.member_div);
 #pragma unroll
-            for (int q = 0; q < P::E; ++q) nx[q] = okn ? src[t + q * P::TPL] : C{0, 0};
+        for (int q = 0; q < P::E; ++q) {
+            const int n0 = foldi(t + q * P::TPL, N);
+            v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
         }
-        if constexpr (OP == OP_FWD) {
-            fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
-        } else if constexpr (OP == OP_INV) {
-            fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
-        } else {
-            fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
-            // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
-            const int k1 = (int)(line % mul.member_div);
-            const MulArgs ma = member_args(mul, line / mul.member_div);
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int n0 = foldi(t + q * P::TPL, N);
-                v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
-            }
-            fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
-        }
-        if (ok) {
-            C *dst = data + line * N;
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) dst[t + q * P::TPL] = v[q];
-        }
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = nx[q];
-        __syncthreads();
+        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
     }
-}
-
-// strided columns: data[o*N*inner + p*inner + col], col in [0, inner); a unit
-// is TC adjacent columns of one outer index
-template <int N, typename R, int OP>  // OP_FWD, OP_INV, OP_DRIFT or OP_POISSON
-__global__ void __launch_bounds__((Geom<N, R>::STRIDED_THREADS))
-strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles, long long nunits,
-            MulArgs mul, const typename cplx<R>::T *__restrict__ twg)
-{
-    using C = typename cplx<R>::T;
-    using P = Plan<N>;
-    using G = Geom<N, R>;
-    constexpr int TC = G::TC;
-    __shared__ C lds[N * TC];
-    __shared__ C tw[N];
-    load_tw<N>(tw, twg);
-    const int c = threadIdx.x % TC, t = threadIdx.x / TC;
-    const long long nb = gridDim.x;
-    long long u = xcd_block();
-    C v[P::E];
-    {
-        const long long o = u / ntiles, col = (u % ntiles) * TC + c;
-        const C *src = data + o * (long long)N * inner + col;
+    if (ok) {
 #pragma unroll
-        for (int q = 0; q < P::E; ++q)
-            v[q] = (u < nunits && col < inner) ? src[(long long)(t + q * P::TPL) * inner] : C{0, 0};
-    }
-    __syncthreads();
-    for (; u < nunits; u += nb) {
-        const long long o = u / ntiles;
-        const long long col = (u % ntiles) * TC + c;
-        const bool ok = col < inner;
-        C nx[P::E];
-        {
-            const long long un = u + nb, on = un / ntiles, cn = (un % ntiles) * TC + c;
-            const bool okn = un < nunits && cn < inner;
-            const C *src = data + on * (long long)N * inner + cn;
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) nx[q] = okn ? src[(long long)(t + q * P::TPL) * inner] : C{0, 0};
-        }
-        if constexpr (OP == OP_FWD) {
-            fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-        } else if constexpr (OP == OP_INV) {
-            fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
-        } else {
-            fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-            const MulArgs ma = member_args(mul, o / mul.member_div);
-            const int cc = ok ? (int)col : 0;
-            if constexpr (OP == OP_DRIFT) {
-                const double2 *T = ma.ptab;
-                double2 cf;
-                if (mul.geo == GEO_PSI3) cf = zmul(T[cc / N], T[cc % N]);
-                else cf = T[cc];
-#pragma unroll
-                for (int q = 0; q < P::E; ++q) v[q] = drift_factor(T[t + q * P::TPL], cf, ma.inv_s, v[q]);
-            } else {  // GEO_VT3 Poisson
-                const int a = foldi(cc, N);
-                const int mc = a * a + (int)(o * o);
-#pragma unroll
-                for (int q = 0; q < P::E; ++q) {
-                    const int n0 = foldi(t + q * P::TPL, N);
-                    v[q] = poisson_factor<R>(ma, n0 * n0 + mc, v[q]);
-                }
-            }
-            fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
-        }
-        if (ok) {
-            C *dst = data + o * (long long)N * inner + col;
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) dst[(long long)(t + q * P::TPL) * inner] = v[q];
-        }
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = nx[q];
-        __syncthreads();
+        for (int q = 0; q < P::E; ++q) src[t + q * P::TPL] = v[q];
     }
 }
 
 // ---------------------------------------------------------------------------
-// real rows -> vt (half spectrum, k-major), two rows per complex transform
+// strided columns: data[o*N*inner + p*inner + col], col in [0, inner); a
+// block takes TC adjacent columns of one outer index.  TC is a tile variant
+// (Geom::TC or half of it): narrower tiles use half the LDS, so twice as
+// many blocks share a CU and overlap their loads with each other's
+// transforms; the variant is picked per pass by measurement.
+// ---------------------------------------------------------------------------
+template <int N, typename R, int OP, int TC>  // OP_FWD, OP_INV, OP_DRIFT or OP_POISSON
+__global__ void __launch_bounds__((TC * Plan<N>::TPL))
+strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
+            MulArgs mul, const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    __shared__ C lds[N * TC];
+    __shared__ C tw[N];
+    // the drift's phase table is staged in LDS when small (N <= 1024)
+    constexpr bool PT_LDS = (OP == OP_DRIFT) && N <= 1024;
+    __shared__ double2 ptab_lds[PT_LDS ? N : 1];
+    load_tw<N>(tw, twg);
+    const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+    const long long b = xcd_block();
+    const long long o = b / ntiles;
+    const long long col = (b % ntiles) * TC + c;
+    const bool ok = col < inner;
+    const MulArgs ma = member_args(mul, o / (mul.member_div > 0 ? mul.member_div : 1));
+    const double2 *ptab = ma.ptab;
+    if constexpr (PT_LDS) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) ptab_lds[i] = ma.ptab[i];
+        ptab = ptab_lds;
+    }
+    C *base = data + o * (long long)N * inner + col;
+    C v[P::E];
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) v[q] = ok ? base[(long long)(t + q * P::TPL) * inner] : C{0, 0};
+    __syncthreads();
+    if constexpr (OP == OP_FWD) {
+        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+    } else if constexpr (OP == OP_INV) {
+        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+    } else {
+        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+        const int cc = ok ? (int)col : 0;
+        if constexpr (OP == OP_DRIFT) {
+            double2 cf;
+            if (mul.geo == GEO_PSI3) cf = zmul(ptab[cc / N], ptab[cc % N]);
+            else cf = ptab[cc];
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) v[q] = drift_factor(ptab[t + q * P::TPL], cf, ma.inv_s, v[q]);
+        } else {  // GEO_VT3 Poisson
+            const int a = foldi(cc, N);
+            const int mc = a * a + (int)(o * o);
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int n0 = foldi(t + q * P::TPL, N);
+                v[q] = poisson_factor<R>(ma, n0 * n0 + mc, v[q]);
+            }
+        }
+        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
+    }
+    if (ok) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) base[(long long)(t + q * P::TPL) * inner] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// real rows -> vt (half spectrum, k-major), two rows per complex transform.
+// Persistent like the transform passes: a unit is TL consecutive rows; the
+// next unit's rows are loaded into registers before the current one is
+// transformed.
 // ---------------------------------------------------------------------------
 enum {
     R2C_REAL = 0,      // input: real rows (vin)
@@ -348,6 +317,35 @@ enum {
     R2C_INV_DENSITY = 2,  // input: psi rows transformed along the last axis;
                           // inverse-transform, store psi, then as DENSITY
 };
+
+template <int N, typename R, int MODE> struct RowIn;
+template <int N, typename R> struct RowIn<N, R, R2C_REAL> {
+    R a[Plan<N>::E], b[Plan<N>::E];
+};
+template <int N, typename R> struct RowIn<N, R, R2C_DENSITY> {
+    typename cplx<R>::T a[Plan<N>::E], b[Plan<N>::E];
+};
+template <int N, typename R> struct RowIn<N, R, R2C_INV_DENSITY> : RowIn<N, R, R2C_DENSITY> {};
+
+template <int N, typename R, int MODE>
+__device__ __forceinline__ void r2c_load(RowIn<N, R, MODE> &in, const typename cplx<R>::T *psi, const R *vin,
+                                         long long ra, long long rb, long long rows, int t)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    const bool oka = ra < rows, okb = rb < rows;
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) {
+        const int pos = t + q * P::TPL;
+        if constexpr (MODE == R2C_REAL) {
+            in.a[q] = oka ? vin[ra * N + pos] : (R)0;
+            in.b[q] = okb ? vin[rb * N + pos] : (R)0;
+        } else {
+            in.a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            in.b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+        }
+    }
+}
 
 template <int N, typename R, int MODE>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
@@ -363,83 +361,85 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
-    const long long r0 = (long long)blockIdx.x * TL;
-    const long long ra = r0 + 2 * p, rb = ra + 1;
-    const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
-    R xa[P::E], xb[P::E];
-    if (MODE == R2C_REAL) {
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            xa[q] = oka ? vin[ra * N + pos] : (R)0;
-            xb[q] = okb ? vin[rb * N + pos] : (R)0;
+    const long long nunits = (rows + TL - 1) / TL, nb = gridDim.x;
+    long long u = xcd_block();
+    RowIn<N, R, MODE> cur;
+    r2c_load<N, R, MODE>(cur, psi, vin, u * TL + 2 * p, u * TL + 2 * p + 1, u < nunits ? rows : 0, t);
+    __syncthreads();
+    for (; u < nunits; u += nb) {
+        const long long r0 = u * TL;
+        const long long ra = r0 + 2 * p, rb = ra + 1;
+        const bool oka = ra < rows, okb = rb < rows;
+        RowIn<N, R, MODE> nxt;
+        {
+            const long long un = u + nb;
+            r2c_load<N, R, MODE>(nxt, psi, vin, un * TL + 2 * p, un * TL + 2 * p + 1, un < nunits ? rows : 0, t);
         }
-        __syncthreads();
-    } else {
-        C a[P::E], b[P::E];
+        R xa[P::E], xb[P::E];
+        if constexpr (MODE == R2C_REAL) {
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
-        }
-        __syncthreads();
-        if (MODE == R2C_INV_DENSITY) {
-            fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
-            fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
+            for (int q = 0; q < P::E; ++q) { xa[q] = cur.a[q]; xb[q] = cur.b[q]; }
+        } else {
+            if constexpr (MODE == R2C_INV_DENSITY) {
+                fft_line<N, +1, LdsContig>(cur.a, t, mine, 1, tw);
+                fft_line<N, +1, LdsContig>(cur.b, t, mine, 1, tw);
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    const int pos = t + q * P::TPL;
+                    if (oka) psi[ra * N + pos] = cur.a[q];
+                    if (okb) psi[rb * N + pos] = cur.b[q];
+                }
+            }
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                if (oka) psi[ra * N + pos] = a[q];
-                if (okb) psi[rb * N + pos] = b[q];
+                xa[q] = pref * (cur.a[q].x * cur.a[q].x + cur.a[q].y * cur.a[q].y - rho_bar);
+                xb[q] = pref * (cur.b[q].x * cur.b[q].x + cur.b[q].y * cur.b[q].y - rho_bar);
             }
         }
+        // z = xa + i xb, Z = FFT(z); Xa[k] = (Z[k] + conj Z[N-k]) / 2,
+        // Xb[k] = (Z[k] - conj Z[N-k]) / (2i)
+        C z[P::E];
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            xa[q] = pref * (a[q].x * a[q].x + a[q].y * a[q].y - rho_bar);
-            xb[q] = pref * (b[q].x * b[q].x + b[q].y * b[q].y - rho_bar);
+        for (int q = 0; q < P::E; ++q) z[q] = C{xa[q], xb[q]};
+        fft_line<N, -1, LdsContig>(z, t, mine, 1, tw);
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = z[q];
+        __syncthreads();
+        C oa[P::E / 2 + 1], ob[P::E / 2 + 1];
+#pragma unroll
+        for (int q = 0; q <= P::E / 2; ++q) {
+            const int k = t + q * P::TPL;
+            if (k <= N / 2) {
+                const C zk = mine[LdsContig::at(k, 1)];
+                const C zm = mine[LdsContig::at((N - k) & (N - 1), 1)];
+                oa[q] = C{(R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y)};
+                ob[q] = C{(R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x)};
+            }
         }
-    }
-    // z = xa + i xb, Z = FFT(z); Xa[k] = (Z[k] + conj Z[N-k]) / 2,
-    // Xb[k] = (Z[k] - conj Z[N-k]) / (2i)
-    C z[P::E];
+        __syncthreads();
+        C *stg = lds;  // staging aliases the exchange area: stg[k*TL + l]
 #pragma unroll
-    for (int q = 0; q < P::E; ++q) z[q] = C{xa[q], xb[q]};
-    fft_line<N, -1, LdsContig>(z, t, mine, 1, tw);
-#pragma unroll
-    for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = z[q];
-    __syncthreads();
-    C oa[P::E / 2 + 1], ob[P::E / 2 + 1];
-#pragma unroll
-    for (int q = 0; q <= P::E / 2; ++q) {
-        const int k = t + q * P::TPL;
-        if (k <= N / 2) {
-            const C zk = mine[LdsContig::at(k, 1)];
-            const C zm = mine[LdsContig::at((N - k) & (N - 1), 1)];
-            oa[q] = C{(R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y)};
-            ob[q] = C{(R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x)};
+        for (int q = 0; q <= P::E / 2; ++q) {
+            const int k = t + q * P::TPL;
+            if (k <= N / 2) {
+                stg[k * TL + 2 * p] = oa[q];
+                stg[k * TL + 2 * p + 1] = ob[q];
+            }
         }
-    }
-    __syncthreads();
-    C *stg = lds;  // staging aliases the exchange area: stg[k*TL + l]
-#pragma unroll
-    for (int q = 0; q <= P::E / 2; ++q) {
-        const int k = t + q * P::TPL;
-        if (k <= N / 2) {
-            stg[k * TL + 2 * p] = oa[q];
-            stg[k * TL + 2 * p + 1] = ob[q];
+        __syncthreads();
+        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+            const int k = i / TL, l = i % TL;
+            if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[i];
         }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
-        const int k = i / TL, l = i % TL;
-        if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[i];
+        cur = nxt;
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// vt (or real V) -> real rows, fused max|V|, kick and forward transform
+// vt (or real V) -> real rows, fused max|V|, kick and forward transform;
+// persistent with the next unit's vt staging values and psi rows prefetched
 // ---------------------------------------------------------------------------
 enum {
     C2R_SRC_VT = 1,     // rows come from vt (else from the real array V)
@@ -449,7 +449,57 @@ enum {
     C2R_FWD = 16,       // forward-transform psi along the last axis
 };
 
-template <int N, typename R>
+// prefetched inputs of one unit: vt staging values (SRC: rows from vt) or
+// the real rows of V, and the psi rows
+template <int N, typename R, bool SRC> struct C2rIn;
+template <int N, typename R> struct C2rIn<N, R, true> {
+    static constexpr int SPT = ((N / 2 + 1) * Geom<N, R>::TL + Geom<N, R>::ROW_THREADS - 1) /
+                               Geom<N, R>::ROW_THREADS;  // staging values per thread
+    typename cplx<R>::T sv[SPT];
+    typename cplx<R>::T a[Plan<N>::E], b[Plan<N>::E];
+};
+template <int N, typename R> struct C2rIn<N, R, false> {
+    typename cplx<R>::T a[Plan<N>::E], b[Plan<N>::E];
+    R va[Plan<N>::E], vb[Plan<N>::E];
+};
+
+template <int N, typename R, bool SRC>
+__device__ __forceinline__ void c2r_load(C2rIn<N, R, SRC> &in, const typename cplx<R>::T *vt, const R *V,
+                                         const typename cplx<R>::T *psi, long long r0, long long rows,
+                                         long long rpm, int flags, int t, int p)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    using G = Geom<N, R>;
+    constexpr int TL = G::TL, H1 = N / 2 + 1;
+    const long long ra = r0 + 2 * p, rb = ra + 1;
+    const bool oka = ra < rows, okb = rb < rows;
+    if constexpr (SRC) {
+#pragma unroll
+        for (int j = 0; j < C2rIn<N, R, SRC>::SPT; ++j) {
+            const int i = threadIdx.x + j * G::ROW_THREADS;
+            const int k = i / TL, l = i % TL;
+            in.sv[j] = (i < H1 * TL && r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            in.va[q] = oka ? V[ra * N + pos] : (R)0;
+            in.vb[q] = okb ? V[rb * N + pos] : (R)0;
+        }
+    }
+    if (flags & (C2R_KICK | C2R_FWD)) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            in.a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            in.b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+        }
+    }
+}
+
+template <int N, typename R, bool SRC>  // SRC: rows from vt (C2R_SRC_VT), else from V
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
@@ -465,56 +515,93 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     __shared__ double wmax[16];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
-    const long long r0 = (long long)blockIdx.x * TL;
-    const long long ra = r0 + 2 * p, rb = ra + 1;
-    const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
-    R va[P::E], vb[P::E];
-    if (flags & C2R_SRC_VT) {
-        C *stg = lds;
-        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
-            const int k = i / TL, l = i % TL;
-            stg[i] = (r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
+    const long long nunits = (rows + TL - 1) / TL, nb = gridDim.x;
+    long long u = xcd_block();
+    double m = 0.0;  // running max|V| of this thread
+    C2rIn<N, R, SRC> cur;
+    c2r_load<N, R, SRC>(cur, vt, V, psi, u * TL, u < nunits ? rows : 0, rpm, flags, t, p);
+    __syncthreads();
+    for (; u < nunits; u += nb) {
+        const long long r0 = u * TL;
+        const long long ra = r0 + 2 * p, rb = ra + 1;
+        const bool oka = ra < rows, okb = rb < rows;
+        C2rIn<N, R, SRC> nxt;
+        {
+            const long long un = u + nb;
+            c2r_load<N, R, SRC>(nxt, vt, V, psi, un * TL, un < nunits ? rows : 0, rpm, flags, t, p);
         }
+        R va[P::E], vb[P::E];
+        if constexpr (SRC) {
+            C *stg = lds;
+#pragma unroll
+            for (int j = 0; j < C2rIn<N, R, SRC>::SPT; ++j) {
+                const int i = threadIdx.x + j * G::ROW_THREADS;
+                if (i < H1 * TL) stg[i] = cur.sv[j];
+            }
+            __syncthreads();
+            C z[P::E];
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int j = t + q * P::TPL;
+                const bool lo = j <= N / 2;
+                const int k = lo ? j : N - j;
+                C xa = stg[k * TL + 2 * p], xb = stg[k * TL + 2 * p + 1];
+                if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
+                // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
+                z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
+            }
+            __syncthreads();
+            fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
+        } else {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) { va[q] = cur.va[q]; vb[q] = cur.vb[q]; }
+        }
+        if (flags & C2R_STORE_V) {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                if (oka) V[ra * N + pos] = va[q];
+                if (okb) V[rb * N + pos] = vb[q];
+            }
+        }
+        if (flags & C2R_VMAX) {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                if (oka) m = fmax(m, (double)fabs(va[q]));
+                if (okb) m = fmax(m, (double)fabs(vb[q]));
+            }
+        }
+        if (flags & (C2R_KICK | C2R_FWD)) {
+            if (flags & C2R_KICK) {
+                const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
+                const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    R sa, ca, sb, cb;
+                    sincosm1_kick(-dta * va[q], &sa, &ca);
+                    sincosm1_kick(-dtb * vb[q], &sb, &cb);
+                    cur.a[q] = rotate(cur.a[q], sa, ca);
+                    cur.b[q] = rotate(cur.b[q], sb, cb);
+                }
+            }
+            if (flags & C2R_FWD) {
+                fft_line<N, -1, LdsContig>(cur.a, t, mine, 1, tw);
+                fft_line<N, -1, LdsContig>(cur.b, t, mine, 1, tw);
+            }
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                if (oka) psi[ra * N + pos] = cur.a[q];
+                if (okb) psi[rb * N + pos] = cur.b[q];
+            }
+        }
+        cur = nxt;
         __syncthreads();
-        C z[P::E];
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int j = t + q * P::TPL;
-            const bool lo = j <= N / 2;
-            const int k = lo ? j : N - j;
-            C xa = stg[k * TL + 2 * p], xb = stg[k * TL + 2 * p + 1];
-            if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
-            // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
-            z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
-        }
-        __syncthreads();
-        fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
-    } else {
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            va[q] = oka ? V[ra * N + pos] : (R)0;
-            vb[q] = okb ? V[rb * N + pos] : (R)0;
-        }
-    }
-    if (flags & C2R_STORE_V) {
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            if (oka) V[ra * N + pos] = va[q];
-            if (okb) V[rb * N + pos] = vb[q];
-        }
     }
     if (flags & C2R_VMAX) {
-        double m = 0.0;
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            if (oka) m = fmax(m, (double)fabs(va[q]));
-            if (okb) m = fmax(m, (double)fabs(vb[q]));
-        }
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
         __syncthreads();
@@ -522,38 +609,6 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) m = fmax(m, wmax[w]);
             // non-negative doubles order like their bit patterns
             atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(m));
-        }
-    }
-    if (flags & (C2R_KICK | C2R_FWD)) {
-        C a[P::E], b[P::E];
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
-        }
-        if (flags & C2R_KICK) {
-            const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
-            const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                R sa, ca, sb, cb;
-                sincosm1_kick(-dta * va[q], &sa, &ca);
-                sincosm1_kick(-dtb * vb[q], &sb, &cb);
-                a[q] = rotate(a[q], sa, ca);
-                b[q] = rotate(b[q], sb, cb);
-            }
-        }
-        if (flags & C2R_FWD) {
-            __syncthreads();
-            fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
-            fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
-        }
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            if (oka) psi[ra * N + pos] = a[q];
-            if (okb) psi[rb * N + pos] = b[q];
         }
     }
 }

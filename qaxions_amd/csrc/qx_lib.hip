@@ -809,6 +809,59 @@ qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last
 
 int qx_step_pass_count(const qx_ctx *c) { return valid_ctx(c) ? step_pass_count(c) : 0; }
 
+// which strided tile-variant slot (op kind, stride class) a step pass uses
+static bool pass_slot(const qx_ctx *c, int p, int *kind, int *cls)
+{
+    if (c->dim == 3) {
+        switch (p) {
+        case 1: *kind = 0; *cls = c->N > 4096 ? 1 : 0; return true;   // fwd axis 1, inner N
+        case 2: *kind = 2; *cls = (long long)c->N * c->N > 4096 ? 1 : 0; return true;  // drift axis 0
+        case 3: *kind = 1; *cls = c->N > 4096 ? 1 : 0; return true;   // inv axis 1
+        case 6: *kind = 3; *cls = c->N > 4096 ? 1 : 0; return true;   // vt Poisson
+        default: return false;
+        }
+    }
+    if (p == 1) { *kind = 2; *cls = c->N > 4096 ? 1 : 0; return true; }
+    return false;
+}
+
+qx_status qx_tune(qx_ctx *c, double ds, int reps, int *variants)
+{
+    CHECK_CTX(c);
+    if (reps < 1) return fail(QX_EINVAL, "reps must be >= 1");
+    const int np = step_pass_count(c);
+    double ms[2][8];
+    for (int v = 0; v < 2; ++v) {
+        for (auto &row : c->eng->strided_variant) row[0] = row[1] = v;
+        TRY(qx_time_step_passes(c, ds, reps, ms[v], nullptr, 8));
+    }
+    for (auto &row : c->eng->strided_variant) row[0] = row[1] = 0;
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured launches used other tiles
+    c->gexec = nullptr;
+    for (int p = 0; p < np; ++p) {
+        int k, cl;
+        if (pass_slot(c, p, &k, &cl)) c->eng->strided_variant[k][cl] = ms[1][p] < ms[0][p] ? 1 : 0;
+    }
+    if (variants)
+        for (int k = 0; k < 4; ++k)
+            for (int cl = 0; cl < 2; ++cl) variants[2 * k + cl] = c->eng->strided_variant[k][cl];
+    return QX_OK;
+}
+
+qx_status qx_set_tile_variants(qx_ctx *c, const int *variants)
+{
+    if (!valid_ctx(c) || !variants) return fail(QX_EINVAL, "null argument");
+    for (int k = 0; k < 4; ++k)
+        for (int cl = 0; cl < 2; ++cl) {
+            if (variants[2 * k + cl] != 0 && variants[2 * k + cl] != 1)
+                return fail(QX_EINVAL, "variants are 0 (full tile) or 1 (half tile)");
+            c->eng->strided_variant[k][cl] = variants[2 * k + cl];
+        }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured launches used the old tiles
+    c->gexec = nullptr;
+    return QX_OK;
+}
+
 const char *qx_step_pass_name(const qx_ctx *c, int p)
 {
     static const char *n3[] = {"c2r_kick_fwd2", "fwd1", "drift0", "inv1", "inv2_density_r2c",
