@@ -85,9 +85,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pass-reps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-variant autotuner")
-    ap.add_argument("--workload", default="kdk", choices=["kdk", "ensemble", "learned"],
+    ap.add_argument("--workload", default="kdk", choices=["kdk", "ensemble", "learned", "copy"],
                     help="kdk: the headline 512^3 step (default); ensemble: 96 uneven 2D "
-                         "lattices; learned: corrector training and evaluation at 512^2")
+                         "lattices; learned: corrector training and evaluation at 512^2; "
+                         "copy: device-to-device state copies of known size (the reference "
+                         "dispatch for scaling HBM traffic counters)")
     ap.add_argument("--train-pairs", type=int, default=20000)
     ap.add_argument("--eval-steps", type=int, default=1000)
     args = ap.parse_args()
@@ -103,6 +105,8 @@ def main():
         return bench_ensemble(qx, args)
     if args.workload == "learned":
         return bench_learned(qx, args)
+    if args.workload == "copy":
+        return bench_copy(qx, args)
 
     N, dim, L, dt = args.N, args.dim, 40.0, 2.5e-4
     S = N ** dim
@@ -136,7 +140,7 @@ def main():
     names, pms, pbytes = lat.time_step_passes(dt, reps=args.pass_reps)
     dom = max(range(len(names)), key=lambda i: pms[i])
     for n_, t_, b_ in zip(names, pms, pbytes):
-        print(f"[bench] pass {n_:>18s}: {t_*1e3:8.1f} us  {b_/t_/1e9:8.1f} GB/s",
+        print(f"[bench] pass {n_:>18s}: {t_*1e3:8.1f} us  {b_/t_/1e6:8.1f} GB/s",
               file=sys.stderr, flush=True)
     csz = 8 if args.precision == "fp32" else 16
     B_site = algorithmic_bytes_per_site(N, c=csz, r=csz // 2)
@@ -182,6 +186,31 @@ def main():
                                          f"(radix-2 FFT, OpenMP) on the same {N}^{dim} lattice and IC, "
                                          f"{t:.1f} s wall"}
     print(json.dumps(out), flush=True)
+
+
+def bench_copy(qx, args):
+    """Copies of one 512^3 FP32 state into another: per copy three runtime
+    copy dispatches, psi (2^27 complex64 = 1 GiB read + 1 GiB written), V
+    (0.5 GiB) and vt (2^26 + 2^18 complex64).  A counter's reading on the psi
+    dispatch over 2 x 1 GiB is the factor that scales it to bytes."""
+    N = args.N
+    a = qx.Lattice(3, N, 40.0, precision="fp32")
+    b = qx.Lattice(3, N, 40.0, precision="fp32")
+    a.ic_grf(seed=0, amp=0.1)
+    reps = 3
+    ev = HipEvents()
+    e0, e1 = ev.event(), ev.event()
+    b.copy_from(a)
+    ev.record(e0, b.stream)
+    for _ in range(reps):
+        b.copy_from(a)
+    ev.record(e1, b.stream)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    S = N ** 3
+    moved = 2 * (8 * S + 4 * S + 8 * (S // N) * (N // 2 + 1))
+    print(json.dumps({"metric": "device-to-device state copy", "value": moved / (ms * 1e-3) / 1e9,
+                      "unit": "GB/s", "ms_per_copy": ms, "bytes_per_copy": moved,
+                      "psi_dispatch_bytes": {"read": 8 * S, "written": 8 * S}}), flush=True)
 
 
 def bench_ensemble(qx, args):
@@ -230,7 +259,7 @@ def bench_learned(qx, args):
     per_seed = 1000
     nseeds = max(1, args.train_pairs // per_seed)
     t0 = time.perf_counter()
-    net, losses = learned.train(N, L, dt, k, seeds=range(1000, 1000 + nseeds),
+    net, losses, relative = learned.train(N, L, dt, k, seeds=range(1000, 1000 + nseeds),
                                 steps_per_seed=min(per_seed, args.train_pairs), lr=1e-3, lr_final=1e-4,
                                 log_every=1000, pref=pref)
     t_train = time.perf_counter() - t0
@@ -239,7 +268,8 @@ def bench_learned(qx, args):
     out = {"metric": "learned coarse stepping at 512^2: speed-up over fine stepping and errors",
            "value": res["speedup_learned_vs_fine"], "unit": "x", "n_gpus": 1,
            "train_pairs": int(len(losses)), "train_seconds": t_train,
-           "loss_first_1k": float(losses[:1000].mean()), "loss_last_1k": float(losses[-1000:].mean()),
+           "relative_loss_first_1k": float(relative[:1000].mean()),
+           "relative_loss_last_1k": float(relative[-1000:].mean()),
            "eval": res, "dtype": "c64 state, bf16 weights, fp32 accumulate",
            "config": {"workload": f"2D {N}^2, L={L}, pref={pref}, fine dt={dt}, coarse {k} dt, "
                                   f"{args.eval_steps} coarse steps, mass tol 1e-6 per 10-step chunk"}}

@@ -202,6 +202,8 @@ qx_status qx_net_loss_grad_host(qx_net *net, const float *x, const float *t, dou
 /* Adam on the last gradient (bias-corrected) */
 qx_status qx_net_adam_step(qx_net *net, float lr, float beta1, float beta2, float eps);
 qx_status qx_net_train_step(qx_net *net, const void *x_dev, const void *t_dev, float lr, double *loss);
+/* mean((x - target)^2): the loss an identity corrector would have */
+qx_status qx_net_pair_mse(qx_net *net, const void *x_dev, const void *t_dev, double *mse);
 /* psi <- psi + net(psi) on a 2D FP32 context of size H x W (V untouched) */
 qx_status qx_net_correct(qx_net *net, qx_ctx *ctx);
 /* nsteps corrected coarse steps of k*dt each (leapfrog(k dt, 1), correct,

@@ -111,6 +111,7 @@ SIGNATURES = {
     "qx_net_adam_step": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
     "qx_net_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, _dp]),
     "qx_net_correct": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qx_net_pair_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     "qx_net_coarse_steps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
                                       C.c_double, C.POINTER(C.c_int)]),
     "qx_net_stream": (C.c_void_p, [C.c_void_p]),
@@ -538,6 +539,15 @@ class Corrector:
         loss = C.c_double()
         _check_net(lib().qx_net_train_step(self._h, pc, pf, float(lr), C.byref(loss)))
         return loss.value
+
+    def pair_mse(self, coarse, fine):
+        """mean((coarse.psi - fine.psi)^2): the identity corrector's loss"""
+        pc, pf = C.c_void_p(), C.c_void_p()
+        _check(lib().qx_device_state(coarse.handle, C.byref(pc), None))
+        _check(lib().qx_device_state(fine.handle, C.byref(pf), None))
+        m = C.c_double()
+        _check_net(lib().qx_net_pair_mse(self._h, pc, pf, C.byref(m)))
+        return m.value
 
     def correct(self, lat):
         _check_net(lib().qx_net_correct(self._h, lat.handle))

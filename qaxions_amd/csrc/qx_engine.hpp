@@ -17,11 +17,14 @@ struct Engine {
     virtual void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                          hipStream_t s) = 0;
     // rows of `rpm` rows per member (rpm = rows for a single lattice)
+    // psit: the k-major copy of psi's rows (R2C_INV_DENSITY_T input,
+    // C2R_FWD_T output); unused otherwise
     virtual void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows,
-                          long long rpm, double pref, double rho_bar, hipStream_t s) = 0;
+                          long long rpm, double pref, double rho_bar, hipStream_t s,
+                          const void *psit = nullptr) = 0;
     virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm,
                           double kick_dt, const double *kick_member, int flags,
-                          unsigned long long *vmax, hipStream_t s) = 0;
+                          unsigned long long *vmax, hipStream_t s, void *psit = nullptr) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
     // tile variant per strided pass kind (0 = full tile, 1 = half tile),
     // indexed by [op kind][stride class]; set by qx_tune
@@ -57,6 +60,7 @@ template <int N, typename R> struct EngineT final : Engine {
         const C *t = (const C *)tw;
         if (op < 0) contig_fft<N, R, OP_FWD><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
         else if (op > 0) contig_fft<N, R, OP_INV><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        else if (m.kind == MUL_DRIFT) contig_fft<N, R, OP_DRIFT><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
         else contig_fft<N, R, OP_POISSON><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
     }
 
@@ -79,37 +83,31 @@ template <int N, typename R> struct EngineT final : Engine {
         if (variant(op, m, inner)) strided_tc<G::TC_HALF>(data, outer, inner, op, m, s);
         else strided_tc<G::TC>(data, outer, inner, op, m, s);
     }
-    long long r2c_cap[3] = {0, 0, 0}, c2r_cap[2] = {0, 0};
-
     void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
-                  double pref, double rho_bar, hipStream_t s) override
+                  double pref, double rho_bar, hipStream_t s, const void *psit) override
     {
-        const long long units = (rows + G::TL - 1) / G::TL;
+        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         C *p = (C *)psi;
         const R *vi = (const R *)vin;
         C *o = (C *)vt;
         const C *t = (const C *)tw;
-        auto launch = [&](auto kernel, long long &cap) {
-            if (!cap) cap = resident_blocks(kernel, G::ROW_THREADS);
-            const unsigned blocks = (unsigned)std::min(units, cap);
-            kernel<<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
-        };
-        if (mode == R2C_REAL) launch(qx::r2c_rows<N, R, R2C_REAL>, r2c_cap[0]);
-        else if (mode == R2C_DENSITY) launch(qx::r2c_rows<N, R, R2C_DENSITY>, r2c_cap[1]);
-        else launch(qx::r2c_rows<N, R, R2C_INV_DENSITY>, r2c_cap[2]);
+        if (mode == R2C_REAL)
+            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
+        else if (mode == R2C_DENSITY)
+            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
+        else if (mode == R2C_INV_DENSITY)
+            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
+        else
+            qx::r2c_rows<N, R, R2C_INV_DENSITY_T><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
-                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s) override
+                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s,
+                  void *psit) override
     {
-        const long long units = (rows + G::TL - 1) / G::TL;
-        auto launch = [&](auto kernel, long long &cap) {
-            if (!cap) cap = resident_blocks(kernel, G::ROW_THREADS);
-            const unsigned blocks = (unsigned)std::min(units, cap);
-            kernel<<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm, (R)kick_dt,
-                                                     kick_member, flags, vmax, (const C *)tw);
-        };
-        if (flags & C2R_SRC_VT) launch(qx::c2r_rows<N, R, true>, c2r_cap[0]);
-        else launch(qx::c2r_rows<N, R, false>, c2r_cap[1]);
+        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
+        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
+                                                             (R)kick_dt, kick_member, flags, vmax,
+                                                             (const C *)tw, (C *)psit);
     }
 };
 

@@ -45,14 +45,20 @@ template <int N, typename R> struct Geom {
     static constexpr int NP0 = (256 / P::TPL) > 8 ? (256 / P::TPL) : 8;
     static constexpr int NP1 = (NP0 * P::TPL > 1024) ? (1024 / P::TPL) : NP0;
     // LDS budget of ~80 KiB: per pair max(exchange line, two staging rows)
-    static constexpr int PAIR_SLOTS = P::PADDED > (N + 2) ? P::PADDED : (N + 2);
+    // per pair: the exchange line, or two rows of k-major staging (2N slots)
+    static constexpr int PAIR_SLOTS = P::PADDED > 2 * N ? P::PADDED : 2 * N;
     static constexpr int NP2 = (81920 - N * (int)sizeof(C)) / (PAIR_SLOTS * (int)sizeof(C));
     static constexpr int NP = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
     static constexpr int TL = 2 * NP;                 // rows per block
     static constexpr int ROW_THREADS = NP * P::TPL;
     static constexpr int XCH = NP * P::PADDED;        // exchange slots
-    static constexpr int STG = (N / 2 + 1) * TL;      // staging slots
-    static constexpr int ROW_LDS = XCH > STG ? XCH : STG;
+    static constexpr int STG = (N / 2 + 1) * TL;      // staging slots (half spectrum)
+    // staging slots for full rows k-major, when they fit beside the twiddles
+    // (else the k-major rows are read and written element by element)
+    static constexpr bool STG_T_LDS = (long long)(N * TL + N) * (long long)sizeof(C) + 1024 <= 163840;
+    static constexpr int STG_T = STG_T_LDS ? N * TL : 0;
+    static constexpr int ROW_LDS0 = XCH > STG ? XCH : STG;
+    static constexpr int ROW_LDS = ROW_LDS0 > STG_T ? ROW_LDS0 : STG_T;
 };
 
 // pointwise factor in fused passes
@@ -68,6 +74,8 @@ enum {
     GEO_PSI2 = 1,  // psi 2D, axis 0 strided, c = k1
     GEO_VT3 = 2,   // vt 3D, axis i0 strided inside plane o = k2, c = k1
     GEO_VT2 = 3,   // vt 2D, contiguous lines along i0, line index = k1
+    GEO_PSIT3 = 4, // psi k2-major [k2][i0][k1]: axis i0 strided in plane o = k2, c = k1
+    GEO_PSIT2 = 5, // psi k1-major [k1][i0]: contiguous lines along i0, line = k1
 };
 
 struct MulArgs {
@@ -197,7 +205,7 @@ __device__ __forceinline__ void load_tw(C *tw, const C *twg)
 // ---------------------------------------------------------------------------
 // contiguous lines: data[line*N + pos], LPB lines per block
 // ---------------------------------------------------------------------------
-template <int N, typename R, int OP>  // OP_FWD, OP_INV or OP_POISSON (2D vt lines)
+template <int N, typename R, int OP>  // OP_FWD, OP_INV, OP_POISSON (2D vt lines), OP_DRIFT (2D psi, k1-major)
 __global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
 contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul,
            const typename cplx<R>::T *__restrict__ twg)
@@ -221,7 +229,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
     } else if constexpr (OP == OP_INV) {
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
-    } else {
+    } else if constexpr (OP == OP_POISSON) {
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
         // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
         const int k1 = (int)(line % mul.member_div);
@@ -231,6 +239,14 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
             const int n0 = foldi(t + q * P::TPL, N);
             v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
         }
+        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
+    } else {  // OP_DRIFT, GEO_PSIT2: position = i0, line = member * N + k1
+        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+        const long long ln = ok ? line : 0;
+        const MulArgs ma = member_args(mul, ln / mul.member_div);
+        const double2 cf = ma.ptab[(int)(ln % mul.member_div)];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = drift_factor(ma.ptab[t + q * P::TPL], cf, ma.inv_s, v[q]);
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
     }
     if (ok) {
@@ -285,6 +301,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         if constexpr (OP == OP_DRIFT) {
             double2 cf;
             if (mul.geo == GEO_PSI3) cf = zmul(ptab[cc / N], ptab[cc % N]);
+            else if (mul.geo == GEO_PSIT3) cf = zmul(ptab[cc], ptab[(int)o]);
             else cf = ptab[cc];
 #pragma unroll
             for (int q = 0; q < P::E; ++q) v[q] = drift_factor(ptab[t + q * P::TPL], cf, ma.inv_s, v[q]);
@@ -306,52 +323,22 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
 }
 
 // ---------------------------------------------------------------------------
-// real rows -> vt (half spectrum, k-major), two rows per complex transform.
-// Persistent like the transform passes: a unit is TL consecutive rows; the
-// next unit's rows are loaded into registers before the current one is
-// transformed.
+// real rows -> vt (half spectrum, k-major), two rows per complex transform
 // ---------------------------------------------------------------------------
 enum {
     R2C_REAL = 0,      // input: real rows (vin)
     R2C_DENSITY = 1,   // input: psi rows, rho = pref (|psi|^2 - rho_bar)
     R2C_INV_DENSITY = 2,  // input: psi rows transformed along the last axis;
                           // inverse-transform, store psi, then as DENSITY
+    R2C_INV_DENSITY_T = 3,  // as INV_DENSITY, the transformed rows read k-major
+                            // from psit[k * rows + row] (per member)
 };
-
-template <int N, typename R, int MODE> struct RowIn;
-template <int N, typename R> struct RowIn<N, R, R2C_REAL> {
-    R a[Plan<N>::E], b[Plan<N>::E];
-};
-template <int N, typename R> struct RowIn<N, R, R2C_DENSITY> {
-    typename cplx<R>::T a[Plan<N>::E], b[Plan<N>::E];
-};
-template <int N, typename R> struct RowIn<N, R, R2C_INV_DENSITY> : RowIn<N, R, R2C_DENSITY> {};
-
-template <int N, typename R, int MODE>
-__device__ __forceinline__ void r2c_load(RowIn<N, R, MODE> &in, const typename cplx<R>::T *psi, const R *vin,
-                                         long long ra, long long rb, long long rows, int t)
-{
-    using C = typename cplx<R>::T;
-    using P = Plan<N>;
-    const bool oka = ra < rows, okb = rb < rows;
-#pragma unroll
-    for (int q = 0; q < P::E; ++q) {
-        const int pos = t + q * P::TPL;
-        if constexpr (MODE == R2C_REAL) {
-            in.a[q] = oka ? vin[ra * N + pos] : (R)0;
-            in.b[q] = okb ? vin[rb * N + pos] : (R)0;
-        } else {
-            in.a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            in.b[q] = okb ? psi[rb * N + pos] : C{0, 0};
-        }
-    }
-}
 
 template <int N, typename R, int MODE>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
          typename cplx<R>::T *__restrict__ vt, long long rows, long long rpm, R pref, R rho_bar,
-         const typename cplx<R>::T *__restrict__ twg)
+         const typename cplx<R>::T *__restrict__ twg, const typename cplx<R>::T *__restrict__ psit)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -361,85 +348,106 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
+    const long long r0 = (long long)blockIdx.x * TL;
+    const long long ra = r0 + 2 * p, rb = ra + 1;
+    const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
-    const long long nunits = (rows + TL - 1) / TL, nb = gridDim.x;
-    long long u = xcd_block();
-    RowIn<N, R, MODE> cur;
-    r2c_load<N, R, MODE>(cur, psi, vin, u * TL + 2 * p, u * TL + 2 * p + 1, u < nunits ? rows : 0, t);
-    __syncthreads();
-    for (; u < nunits; u += nb) {
-        const long long r0 = u * TL;
-        const long long ra = r0 + 2 * p, rb = ra + 1;
-        const bool oka = ra < rows, okb = rb < rows;
-        RowIn<N, R, MODE> nxt;
-        {
-            const long long un = u + nb;
-            r2c_load<N, R, MODE>(nxt, psi, vin, un * TL + 2 * p, un * TL + 2 * p + 1, un < nunits ? rows : 0, t);
+    R xa[P::E], xb[P::E];
+    if (MODE == R2C_REAL) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            xa[q] = oka ? vin[ra * N + pos] : (R)0;
+            xb[q] = okb ? vin[rb * N + pos] : (R)0;
         }
-        R xa[P::E], xb[P::E];
-        if constexpr (MODE == R2C_REAL) {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) { xa[q] = cur.a[q]; xb[q] = cur.b[q]; }
-        } else {
-            if constexpr (MODE == R2C_INV_DENSITY) {
-                fft_line<N, +1, LdsContig>(cur.a, t, mine, 1, tw);
-                fft_line<N, +1, LdsContig>(cur.b, t, mine, 1, tw);
-#pragma unroll
-                for (int q = 0; q < P::E; ++q) {
-                    const int pos = t + q * P::TPL;
-                    if (oka) psi[ra * N + pos] = cur.a[q];
-                    if (okb) psi[rb * N + pos] = cur.b[q];
-                }
-            }
+        __syncthreads();
+    } else {
+        C a[P::E], b[P::E];
+        if constexpr (MODE == R2C_INV_DENSITY_T && !G::STG_T_LDS) {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
-                xa[q] = pref * (cur.a[q].x * cur.a[q].x + cur.a[q].y * cur.a[q].y - rho_bar);
-                xb[q] = pref * (cur.b[q].x * cur.b[q].x + cur.b[q].y * cur.b[q].y - rho_bar);
+                const int pos = t + q * P::TPL;
+                a[q] = oka ? psit[vt_index(ra, pos, rpm, N)] : C{0, 0};
+                b[q] = okb ? psit[vt_index(rb, pos, rpm, N)] : C{0, 0};
             }
-        }
-        // z = xa + i xb, Z = FFT(z); Xa[k] = (Z[k] + conj Z[N-k]) / 2,
-        // Xb[k] = (Z[k] - conj Z[N-k]) / (2i)
-        C z[P::E];
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) z[q] = C{xa[q], xb[q]};
-        fft_line<N, -1, LdsContig>(z, t, mine, 1, tw);
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = z[q];
-        __syncthreads();
-        C oa[P::E / 2 + 1], ob[P::E / 2 + 1];
-#pragma unroll
-        for (int q = 0; q <= P::E / 2; ++q) {
-            const int k = t + q * P::TPL;
-            if (k <= N / 2) {
-                const C zk = mine[LdsContig::at(k, 1)];
-                const C zm = mine[LdsContig::at((N - k) & (N - 1), 1)];
-                oa[q] = C{(R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y)};
-                ob[q] = C{(R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x)};
+        } else if constexpr (MODE == R2C_INV_DENSITY_T) {
+            // k-major rows: stage [k][l] for the block's TL rows, then gather
+            C *stg = lds;
+            for (int i = threadIdx.x; i < N * TL; i += blockDim.x) {
+                const int k = i / TL, l = i % TL;
+                stg[i] = (r0 + l < rows) ? psit[vt_index(r0 + l, k, rpm, N)] : C{0, 0};
             }
-        }
-        __syncthreads();
-        C *stg = lds;  // staging aliases the exchange area: stg[k*TL + l]
+            __syncthreads();
 #pragma unroll
-        for (int q = 0; q <= P::E / 2; ++q) {
-            const int k = t + q * P::TPL;
-            if (k <= N / 2) {
-                stg[k * TL + 2 * p] = oa[q];
-                stg[k * TL + 2 * p + 1] = ob[q];
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                a[q] = stg[pos * TL + 2 * p];
+                b[q] = stg[pos * TL + 2 * p + 1];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+                b[q] = okb ? psi[rb * N + pos] : C{0, 0};
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
-            const int k = i / TL, l = i % TL;
-            if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[i];
+        if (MODE == R2C_INV_DENSITY || MODE == R2C_INV_DENSITY_T) {
+            fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
+            fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                if (oka) psi[ra * N + pos] = a[q];
+                if (okb) psi[rb * N + pos] = b[q];
+            }
         }
-        cur = nxt;
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            xa[q] = pref * (a[q].x * a[q].x + a[q].y * a[q].y - rho_bar);
+            xb[q] = pref * (b[q].x * b[q].x + b[q].y * b[q].y - rho_bar);
+        }
+    }
+    // z = xa + i xb, Z = FFT(z); Xa[k] = (Z[k] + conj Z[N-k]) / 2,
+    // Xb[k] = (Z[k] - conj Z[N-k]) / (2i)
+    C z[P::E];
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) z[q] = C{xa[q], xb[q]};
+    fft_line<N, -1, LdsContig>(z, t, mine, 1, tw);
+#pragma unroll
+    for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = z[q];
+    __syncthreads();
+    C oa[P::E / 2 + 1], ob[P::E / 2 + 1];
+#pragma unroll
+    for (int q = 0; q <= P::E / 2; ++q) {
+        const int k = t + q * P::TPL;
+        if (k <= N / 2) {
+            const C zk = mine[LdsContig::at(k, 1)];
+            const C zm = mine[LdsContig::at((N - k) & (N - 1), 1)];
+            oa[q] = C{(R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y)};
+            ob[q] = C{(R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x)};
+        }
+    }
+    __syncthreads();
+    C *stg = lds;  // staging aliases the exchange area: stg[k*TL + l]
+#pragma unroll
+    for (int q = 0; q <= P::E / 2; ++q) {
+        const int k = t + q * P::TPL;
+        if (k <= N / 2) {
+            stg[k * TL + 2 * p] = oa[q];
+            stg[k * TL + 2 * p + 1] = ob[q];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+        const int k = i / TL, l = i % TL;
+        if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[i];
     }
 }
 
 // ---------------------------------------------------------------------------
-// vt (or real V) -> real rows, fused max|V|, kick and forward transform;
-// persistent with the next unit's vt staging values and psi rows prefetched
+// vt (or real V) -> real rows, fused max|V|, kick and forward transform
 // ---------------------------------------------------------------------------
 enum {
     C2R_SRC_VT = 1,     // rows come from vt (else from the real array V)
@@ -447,64 +455,16 @@ enum {
     C2R_VMAX = 4,       // fold max|V| into *vmax_bits
     C2R_KICK = 8,       // psi <- psi exp(-i kick_dt V)
     C2R_FWD = 16,       // forward-transform psi along the last axis
+    C2R_FWD_T = 32,     // with C2R_FWD: store the transformed rows k-major to psit
 };
 
-// prefetched inputs of one unit: vt staging values (SRC: rows from vt) or
-// the real rows of V, and the psi rows
-template <int N, typename R, bool SRC> struct C2rIn;
-template <int N, typename R> struct C2rIn<N, R, true> {
-    static constexpr int SPT = ((N / 2 + 1) * Geom<N, R>::TL + Geom<N, R>::ROW_THREADS - 1) /
-                               Geom<N, R>::ROW_THREADS;  // staging values per thread
-    typename cplx<R>::T sv[SPT];
-    typename cplx<R>::T a[Plan<N>::E], b[Plan<N>::E];
-};
-template <int N, typename R> struct C2rIn<N, R, false> {
-    typename cplx<R>::T a[Plan<N>::E], b[Plan<N>::E];
-    R va[Plan<N>::E], vb[Plan<N>::E];
-};
-
-template <int N, typename R, bool SRC>
-__device__ __forceinline__ void c2r_load(C2rIn<N, R, SRC> &in, const typename cplx<R>::T *vt, const R *V,
-                                         const typename cplx<R>::T *psi, long long r0, long long rows,
-                                         long long rpm, int flags, int t, int p)
-{
-    using C = typename cplx<R>::T;
-    using P = Plan<N>;
-    using G = Geom<N, R>;
-    constexpr int TL = G::TL, H1 = N / 2 + 1;
-    const long long ra = r0 + 2 * p, rb = ra + 1;
-    const bool oka = ra < rows, okb = rb < rows;
-    if constexpr (SRC) {
-#pragma unroll
-        for (int j = 0; j < C2rIn<N, R, SRC>::SPT; ++j) {
-            const int i = threadIdx.x + j * G::ROW_THREADS;
-            const int k = i / TL, l = i % TL;
-            in.sv[j] = (i < H1 * TL && r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            in.va[q] = oka ? V[ra * N + pos] : (R)0;
-            in.vb[q] = okb ? V[rb * N + pos] : (R)0;
-        }
-    }
-    if (flags & (C2R_KICK | C2R_FWD)) {
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            in.a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            in.b[q] = okb ? psi[rb * N + pos] : C{0, 0};
-        }
-    }
-}
-
-template <int N, typename R, bool SRC>  // SRC: rows from vt (C2R_SRC_VT), else from V
+template <int N, typename R>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
          const double *__restrict__ kick_member, int flags,
-         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg)
+         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg,
+         typename cplx<R>::T *__restrict__ psit)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -515,93 +475,56 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     __shared__ double wmax[16];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
+    const long long r0 = (long long)blockIdx.x * TL;
+    const long long ra = r0 + 2 * p, rb = ra + 1;
+    const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
-    const long long nunits = (rows + TL - 1) / TL, nb = gridDim.x;
-    long long u = xcd_block();
-    double m = 0.0;  // running max|V| of this thread
-    C2rIn<N, R, SRC> cur;
-    c2r_load<N, R, SRC>(cur, vt, V, psi, u * TL, u < nunits ? rows : 0, rpm, flags, t, p);
-    __syncthreads();
-    for (; u < nunits; u += nb) {
-        const long long r0 = u * TL;
-        const long long ra = r0 + 2 * p, rb = ra + 1;
-        const bool oka = ra < rows, okb = rb < rows;
-        C2rIn<N, R, SRC> nxt;
-        {
-            const long long un = u + nb;
-            c2r_load<N, R, SRC>(nxt, vt, V, psi, un * TL, un < nunits ? rows : 0, rpm, flags, t, p);
+    R va[P::E], vb[P::E];
+    if (flags & C2R_SRC_VT) {
+        C *stg = lds;
+        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+            const int k = i / TL, l = i % TL;
+            stg[i] = (r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
         }
-        R va[P::E], vb[P::E];
-        if constexpr (SRC) {
-            C *stg = lds;
-#pragma unroll
-            for (int j = 0; j < C2rIn<N, R, SRC>::SPT; ++j) {
-                const int i = threadIdx.x + j * G::ROW_THREADS;
-                if (i < H1 * TL) stg[i] = cur.sv[j];
-            }
-            __syncthreads();
-            C z[P::E];
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int j = t + q * P::TPL;
-                const bool lo = j <= N / 2;
-                const int k = lo ? j : N - j;
-                C xa = stg[k * TL + 2 * p], xb = stg[k * TL + 2 * p + 1];
-                if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
-                // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
-                z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
-            }
-            __syncthreads();
-            fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
-        } else {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) { va[q] = cur.va[q]; vb[q] = cur.vb[q]; }
-        }
-        if (flags & C2R_STORE_V) {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                if (oka) V[ra * N + pos] = va[q];
-                if (okb) V[rb * N + pos] = vb[q];
-            }
-        }
-        if (flags & C2R_VMAX) {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                if (oka) m = fmax(m, (double)fabs(va[q]));
-                if (okb) m = fmax(m, (double)fabs(vb[q]));
-            }
-        }
-        if (flags & (C2R_KICK | C2R_FWD)) {
-            if (flags & C2R_KICK) {
-                const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
-                const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
-#pragma unroll
-                for (int q = 0; q < P::E; ++q) {
-                    R sa, ca, sb, cb;
-                    sincosm1_kick(-dta * va[q], &sa, &ca);
-                    sincosm1_kick(-dtb * vb[q], &sb, &cb);
-                    cur.a[q] = rotate(cur.a[q], sa, ca);
-                    cur.b[q] = rotate(cur.b[q], sb, cb);
-                }
-            }
-            if (flags & C2R_FWD) {
-                fft_line<N, -1, LdsContig>(cur.a, t, mine, 1, tw);
-                fft_line<N, -1, LdsContig>(cur.b, t, mine, 1, tw);
-            }
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                if (oka) psi[ra * N + pos] = cur.a[q];
-                if (okb) psi[rb * N + pos] = cur.b[q];
-            }
-        }
-        cur = nxt;
         __syncthreads();
+        C z[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int j = t + q * P::TPL;
+            const bool lo = j <= N / 2;
+            const int k = lo ? j : N - j;
+            C xa = stg[k * TL + 2 * p], xb = stg[k * TL + 2 * p + 1];
+            if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
+            // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
+            z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
+        }
+        __syncthreads();
+        fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
+    } else {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            va[q] = oka ? V[ra * N + pos] : (R)0;
+            vb[q] = okb ? V[rb * N + pos] : (R)0;
+        }
+    }
+    if (flags & C2R_STORE_V) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            if (oka) V[ra * N + pos] = va[q];
+            if (okb) V[rb * N + pos] = vb[q];
+        }
     }
     if (flags & C2R_VMAX) {
+        double m = 0.0;
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            if (oka) m = fmax(m, (double)fabs(va[q]));
+            if (okb) m = fmax(m, (double)fabs(vb[q]));
+        }
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
         __syncthreads();
@@ -609,6 +532,62 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) m = fmax(m, wmax[w]);
             // non-negative doubles order like their bit patterns
             atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+    if (flags & (C2R_KICK | C2R_FWD)) {
+        C a[P::E], b[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+        }
+        if (flags & C2R_KICK) {
+            const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
+            const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                R sa, ca, sb, cb;
+                sincosm1_kick(-dta * va[q], &sa, &ca);
+                sincosm1_kick(-dtb * vb[q], &sb, &cb);
+                a[q] = rotate(a[q], sa, ca);
+                b[q] = rotate(b[q], sb, cb);
+            }
+        }
+        if (flags & C2R_FWD) {
+            __syncthreads();
+            fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
+            fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
+        }
+        if ((flags & C2R_FWD_T) && !G::STG_T_LDS) {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                if (oka) psit[vt_index(ra, pos, rpm, N)] = a[q];
+                if (okb) psit[vt_index(rb, pos, rpm, N)] = b[q];
+            }
+        } else if (G::STG_T_LDS && (flags & C2R_FWD_T)) {
+            // k-major store: stage [k][l], then TL consecutive rows per k
+            __syncthreads();
+            C *stg = lds;
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                stg[pos * TL + 2 * p] = a[q];
+                stg[pos * TL + 2 * p + 1] = b[q];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < N * TL; i += blockDim.x) {
+                const int k = i / TL, l = i % TL;
+                if (r0 + l < rows) psit[vt_index(r0 + l, k, rpm, N)] = stg[i];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                if (oka) psi[ra * N + pos] = a[q];
+                if (okb) psi[rb * N + pos] = b[q];
+            }
         }
     }
 }

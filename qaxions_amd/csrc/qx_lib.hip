@@ -4,13 +4,18 @@
 // and Field's operators (src/field.cxx:93-402); the fused schedule of
 // qx_leapfrog merges neighbouring operators into shared HBM passes:
 //
-//   3D step:  c2r_rows[vt -> V, kick, fwd axis 2]      (K + start of D)
-//             strided fwd axis 1
-//             strided axis 0: fwd, e^{-i dt k^2/2}/S, inv   (D)
-//             strided inv axis 1
-//             r2c_rows[inv axis 2, store psi, density, r2c] (end of D, P)
+//   3D step:  c2r_rows[vt -> V, kick, fwd axis 2, store k2-major psit]
+//                                                       (K + start of D)
+//             contig fwd axis 1 on psit = [k2][i0][i1]
+//             strided axis 0: fwd, e^{-i dt k^2/2}/S, inv (stride N)  (D)
+//             contig inv axis 1
+//             r2c_rows[gather psit rows, inv axis 2, store psi, density,
+//                      r2c]                              (end of D, P)
 //             contig fwd / strided Poisson / contig inv on vt (P)
-//   2D step:  c2r_rows / strided drift / r2c_rows / contig Poisson.
+//   2D step:  c2r_rows[.., store k1-major psit] / contig drift on psit /
+//             r2c_rows[gather psit] / contig Poisson.
+// The k-major copy psit turns the leading axis' transform, which would stride
+// N^2 elements in psi's own layout, into contiguous or stride-N lines.
 #include "../../include/qx.h"
 #include "qx_aux.hpp"
 #include "qx_engine.hpp"
@@ -108,6 +113,7 @@ struct qx_ctx {
     long long S = 0, rows = 0, H1 = 0, H = 0;
     size_t csz = 0, rsz = 0;
     void *psi = nullptr, *V = nullptr, *vt = nullptr;
+    void *psit = nullptr;    // k-major copy of psi inside a fused step (lazy)
     void *aux_c = nullptr;   // S complex scratch (lazy)
     void *aux_vt = nullptr;  // H complex scratch (lazy)
     void *aux_r = nullptr;   // S real scratch (lazy)
@@ -238,7 +244,9 @@ void fill_phase_table(double2 *T, int N, double L, double dt)
     }
 }
 
-qx_status drift_args(qx_ctx *c, double dt, MulArgs *m)
+// kmajor: the factor applied on the k-major copy psit (fused step) rather
+// than on psi in its own layout (qx_drift)
+qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool kmajor = false)
 {
     if (!c->ptab) TRY(ensure(c, (void **)&c->ptab, sizeof(double2) * c->N));
     if (!(c->ptab_dt == dt)) {
@@ -250,11 +258,11 @@ qx_status drift_args(qx_ctx *c, double dt, MulArgs *m)
     }
     *m = MulArgs{};
     m->kind = MUL_DRIFT;
-    m->geo = c->dim == 3 ? GEO_PSI3 : GEO_PSI2;
+    m->geo = kmajor ? (c->dim == 3 ? GEO_PSIT3 : GEO_PSIT2) : (c->dim == 3 ? GEO_PSI3 : GEO_PSI2);
     m->ptab = c->ptab;
     m->ptab_stride = 0;
     m->inv_s = 1.0 / (double)c->S;
-    m->member_div = 1;
+    m->member_div = (kmajor && c->dim == 2) ? c->N : 1;
     return QX_OK;
 }
 MulArgs poisson_args(const qx_ctx *c)
@@ -394,28 +402,32 @@ bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
 // ---- the fused steady-state step, pass by pass ---------------------------
 int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 8 : 4; }
 
-// pass p of one fused KDK step acting on (psi, vt); see the file header
-void step_pass(qx_ctx *c, int p, double ds, void *psi, void *vt, const MulArgs &dm, const MulArgs &pm)
+// pass p of one fused KDK step acting on psi (real space, row layout), its
+// k-major copy psit and vt; see the file header.  dm is drift_args(kmajor)
+void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, const MulArgs &dm,
+               const MulArgs &pm)
 {
     MulArgs none{};
     const int N = c->N;
-    const long long NN = (long long)N * N;
+    const int kick = C2R_SRC_VT | C2R_KICK | C2R_FWD | C2R_FWD_T;
     if (c->dim == 3) {
+        // psit = [k2][i0][i1]: the i1 passes are contiguous, the i0 pass N-strided
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
-        case 1: c->eng->strided(psi, N, N, -1, none, c->stream); break;
-        case 2: c->eng->strided(psi, 1, NN, 0, dm, c->stream); break;
-        case 3: c->eng->strided(psi, N, N, +1, none, c->stream); break;
-        case 4: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, psit); break;
+        case 1: c->eng->contig(psit, c->rows, -1, none, c->stream); break;
+        case 2: c->eng->strided(psit, N, N, 0, dm, c->stream); break;
+        case 3: c->eng->contig(psit, c->rows, +1, none, c->stream); break;
+        case 4: c->eng->r2c_rows(R2C_INV_DENSITY_T, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream, psit); break;
         case 5: c->eng->contig(vt, c->H1 * N, -1, none, c->stream); break;
         case 6: c->eng->strided(vt, c->H1, N, 0, pm, c->stream); break;
         case 7: c->eng->contig(vt, c->H1 * N, +1, none, c->stream); break;
         }
     } else {
+        // psit = [k1][i0]: the i0 drift pass is contiguous
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, C2R_SRC_VT | C2R_KICK | C2R_FWD, nullptr, c->stream); break;
-        case 1: c->eng->strided(psi, 1, N, 0, dm, c->stream); break;
-        case 2: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, psit); break;
+        case 1: c->eng->contig(psit, c->rows, 0, dm, c->stream); break;
+        case 2: c->eng->r2c_rows(R2C_INV_DENSITY_T, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream, psit); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
     }
@@ -521,7 +533,7 @@ qx_status qx_destroy(qx_ctx *c)
     if (!c) return QX_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->psi, c->V, c->vt, c->aux_c, c->aux_vt, c->aux_r, c->stage,
+    void *bufs[] = {c->psi, c->V, c->vt, c->psit, c->aux_c, c->aux_vt, c->aux_r, c->stage,
                     c->partials, c->vmax_bits, c->spec_P, c->spec_cnt, c->ptab,
                     c->eng ? c->eng->tw : nullptr};
     for (void *b : bufs)
@@ -759,7 +771,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, co
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < GRAPH_STEPS; ++i)
-                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
+                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, dm, pm);
             HIPCHK(hipStreamEndCapture(c->stream, &g));
             const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -770,7 +782,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, co
         for (; done + GRAPH_STEPS <= n; done += GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     }
     for (; done < n; ++done)
-        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
+        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, dm, pm);
     return QX_OK;
 }
 
@@ -786,12 +798,13 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
     }
     const MulArgs pm = poisson_args(c);
     MulArgs dm;
-    TRY(drift_args(c, ds, &dm));
+    TRY(drift_args(c, ds, &dm, true));
+    TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     const int np = step_pass_count(c);
     // the opening kick reads the stored V; later kicks the potential's spectrum
-    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->rows, k_first, nullptr, C2R_KICK | C2R_FWD, nullptr,
-                     c->stream);
-    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->vt, dm, pm);
+    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->rows, k_first, nullptr, C2R_KICK | C2R_FWD | C2R_FWD_T,
+                     nullptr, c->stream, c->psit);
+    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, dm, pm);
     TRY(steady_steps(c, nsteps - 1, ds, dm, pm));
     // last step's c2r: materialise V and max|V|, closing half kick
     if (wait) TRY(reset_vmax(c));
@@ -814,14 +827,11 @@ static bool pass_slot(const qx_ctx *c, int p, int *kind, int *cls)
 {
     if (c->dim == 3) {
         switch (p) {
-        case 1: *kind = 0; *cls = c->N > 4096 ? 1 : 0; return true;   // fwd axis 1, inner N
-        case 2: *kind = 2; *cls = (long long)c->N * c->N > 4096 ? 1 : 0; return true;  // drift axis 0
-        case 3: *kind = 1; *cls = c->N > 4096 ? 1 : 0; return true;   // inv axis 1
+        case 2: *kind = 2; *cls = c->N > 4096 ? 1 : 0; return true;   // drift along i0, inner N
         case 6: *kind = 3; *cls = c->N > 4096 ? 1 : 0; return true;   // vt Poisson
         default: return false;
         }
     }
-    if (p == 1) { *kind = 2; *cls = c->N > 4096 ? 1 : 0; return true; }
     return false;
 }
 
@@ -864,9 +874,9 @@ qx_status qx_set_tile_variants(qx_ctx *c, const int *variants)
 
 const char *qx_step_pass_name(const qx_ctx *c, int p)
 {
-    static const char *n3[] = {"c2r_kick_fwd2", "fwd1", "drift0", "inv1", "inv2_density_r2c",
+    static const char *n3[] = {"c2r_kick_fwd2_T", "fwd1", "drift0", "inv1", "inv2_T_density_r2c",
                                "vt_fwd1", "vt_poisson0", "vt_inv1"};
-    static const char *n2[] = {"c2r_kick_fwd1", "drift0", "inv1_density_r2c", "vt_poisson0"};
+    static const char *n2[] = {"c2r_kick_fwd1_T", "drift0", "inv1_T_density_r2c", "vt_poisson0"};
     if (!valid_ctx(c) || p < 0 || p >= step_pass_count(c)) return "";
     return c->dim == 3 ? n3[p] : n2[p];
 }
@@ -882,15 +892,16 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     HIPCHK(hipMemcpyAsync(c->aux_vt, c->vt, (size_t)c->H * c->csz, hipMemcpyDeviceToDevice, c->stream));
     const MulArgs pm = poisson_args(c);
     MulArgs dm;
-    TRY(drift_args(c, ds, &dm));
+    TRY(drift_args(c, ds, &dm, true));
+    TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     qx_status st = QX_OK;
     for (int p = 0; p < np && st == QX_OK; ++p) {
-        step_pass(c, p, ds, c->aux_c, c->aux_vt, dm, pm);  // warm
+        step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, dm, pm);  // warm
         if (hipEventRecord(e0, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
-        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->aux_vt, dm, pm);
+        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, dm, pm);
         if (hipEventRecord(e1, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
         float t = 0.f;
         if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
@@ -1103,6 +1114,7 @@ struct EnsClass {
     int N = 0, B = 0;
     long long S = 0, rows = 0, H1 = 0, H = 0;
     void *psi = nullptr, *V = nullptr, *vt = nullptr;
+    void *psit = nullptr;                                      // k1-major copy of psi, [B][k1][i0]
     double2 *d_ptab = nullptr;                                 // per member phase tables [B][N]
     double *d_pfac = nullptr;                                  // per member
     double *d_kick_first = nullptr, *d_kick_mid = nullptr, *d_kick_last = nullptr;
@@ -1172,11 +1184,11 @@ MulArgs ens_drift_args(const EnsClass &k)
 {
     MulArgs m{};
     m.kind = MUL_DRIFT;
-    m.geo = GEO_PSI2;
+    m.geo = GEO_PSIT2;
     m.inv_s = 1.0 / (double)k.S;
     m.ptab = k.d_ptab;
     m.ptab_stride = k.N;
-    m.member_div = 1;
+    m.member_div = k.N;   // drift lines are member * N + k1
     return m;
 }
 MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
@@ -1194,10 +1206,10 @@ MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
 void ens_class_step(qx_ens *e, EnsClass &k, hipStream_t s)
 {
     const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
-    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_FWD,
-                    nullptr, s);
-    k.eng->strided(k.psi, k.B, k.N, 0, dm, s);
-    k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
+    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid,
+                    C2R_SRC_VT | C2R_KICK | C2R_FWD | C2R_FWD_T, nullptr, s, k.psit);
+    k.eng->contig(k.psit, k.rows, 0, dm, s);
+    k.eng->r2c_rows(R2C_INV_DENSITY_T, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s, k.psit);
     k.eng->contig(k.vt, k.B * k.H1, 0, pm, s);
 }
 
@@ -1307,12 +1319,12 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
             hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)
             return cleanup(fail(QX_EDEVICE, "stream creation failed"));
         void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_ptab, (void **)&k.d_pfac,
-                         (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
+                         (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last, &k.psit};
         const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
                              (size_t)(k.B * k.H) * e->csz, (size_t)k.N * e->csz,
                              sizeof(double2) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
-                             sizeof(double) * k.B, sizeof(double) * k.B};
-        for (int b = 0; b < 9; ++b)
+                             sizeof(double) * k.B, sizeof(double) * k.B, (size_t)(k.B * k.S) * e->csz};
+        for (int b = 0; b < 10; ++b)
             if (hipMalloc(bufs[b], sz[b]) != hipSuccess) {
                 *bufs[b] = nullptr;
                 return cleanup(fail(QX_ENOMEM, "ensemble allocation failed"));
@@ -1338,7 +1350,7 @@ qx_status qx_ens_destroy(qx_ens *e)
     for (auto &k : e->classes) {
         if (k.stream) (void)hipStreamSynchronize(k.stream);
         void *bufs[] = {k.psi, k.V, k.vt, k.eng ? k.eng->tw : nullptr, k.d_ptab, k.d_pfac,
-                        k.d_kick_first, k.d_kick_mid, k.d_kick_last};
+                        k.d_kick_first, k.d_kick_mid, k.d_kick_last, k.psit};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
         if (k.done) (void)hipEventDestroy(k.done);
@@ -1474,9 +1486,11 @@ qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_hal
         for (int s = 0; s < k.B; ++s) kf[s] = (first_half ? 0.5 : 1.0) * ds[k.members[s]];
         HIPCHK(hipMemcpy(k.d_kick_first, kf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
-        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_FWD, nullptr, e->stream);
-        k.eng->strided(k.psi, k.B, k.N, 0, dm, e->stream);
-        k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
+        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_FWD | C2R_FWD_T,
+                        nullptr, e->stream, k.psit);
+        k.eng->contig(k.psit, k.rows, 0, dm, e->stream);
+        k.eng->r2c_rows(R2C_INV_DENSITY_T, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream,
+                        k.psit);
         k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
     }
     TRY(ens_steady_steps(e, nsteps - 1));

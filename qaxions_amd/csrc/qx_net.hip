@@ -123,30 +123,64 @@ conv_vec(const IN *__restrict__ in, int H, int W, const bf16_t *__restrict__ wk,
         ww -= (ww >= W) ? W : 0;
         const IN *src = in + ((long long)hh * W + ww) * CI;
         const float *wt = ws + tap * CI * CO;
-#pragma unroll 4
-        for (int c = 0; c < CI; ++c) {
-            float x;
-            if constexpr (sizeof(IN) == 2) x = bf2f(src[c]);
-            else x = (float)src[c];
+        // the pixel's CI inputs in 16-byte loads
+        constexpr int VEC = 16 / (int)sizeof(IN);
+        static_assert(CI % VEC == 0 || CI < VEC, "channel count must fill 16-byte loads");
+        if constexpr (CI >= VEC) {
 #pragma unroll
-            for (int o = 0; o < CO; ++o) acc[o] = fmaf(wt[c * CO + o], x, acc[o]);
+            for (int c0 = 0; c0 < CI; c0 += VEC) {
+                const uint4 raw = *(const uint4 *)(src + c0);
+                const IN *xv = (const IN *)&raw;
+#pragma unroll
+                for (int cc = 0; cc < VEC; ++cc) {
+                    float x;
+                    if constexpr (sizeof(IN) == 2) x = bf2f(xv[cc]);
+                    else x = (float)xv[cc];
+#pragma unroll
+                    for (int o = 0; o < CO; ++o) acc[o] = fmaf(wt[(c0 + cc) * CO + o], x, acc[o]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < CI; ++c) {
+                float x;
+                if constexpr (sizeof(IN) == 2) x = bf2f(src[c]);
+                else x = (float)src[c];
+#pragma unroll
+                for (int o = 0; o < CO; ++o) acc[o] = fmaf(wt[c * CO + o], x, acc[o]);
+            }
         }
     }
-    if (MODE == 0) {
+    if constexpr (MODE == 0) {
+        // packed 16-byte stores of the pixel's CO outputs
+        static_assert(CO % 8 == 0, "hidden layers have a multiple of 8 channels");
+        if (z_out) {
+#pragma unroll
+            for (int o = 0; o < CO; o += 4)
+                *(float4 *)(z_out + p * CO + o) = float4{acc[o], acc[o + 1], acc[o + 2], acc[o + 3]};
+        }
         bf16_t *a = (bf16_t *)out;
 #pragma unroll
-        for (int o = 0; o < CO; ++o) {
-            if (z_out) z_out[p * CO + o] = acc[o];
-            a[p * CO + o] = f2bf(gelu(acc[o]));
+        for (int o = 0; o < CO; o += 8) {
+            uint4 pk;
+            pk.x = (unsigned)f2bf(gelu(acc[o])) | ((unsigned)f2bf(gelu(acc[o + 1])) << 16);
+            pk.y = (unsigned)f2bf(gelu(acc[o + 2])) | ((unsigned)f2bf(gelu(acc[o + 3])) << 16);
+            pk.z = (unsigned)f2bf(gelu(acc[o + 4])) | ((unsigned)f2bf(gelu(acc[o + 5])) << 16);
+            pk.w = (unsigned)f2bf(gelu(acc[o + 6])) | ((unsigned)f2bf(gelu(acc[o + 7])) << 16);
+            *(uint4 *)(a + p * CO + o) = pk;
         }
-    } else if (MODE == 1) {
+    } else if constexpr (MODE == 1) {
         float *y = (float *)out;
 #pragma unroll
         for (int o = 0; o < CO; ++o) y[p * CO + o] = fmaf(oscale, acc[o], x_res[p * CO + o]);
-    } else if (MODE == 2) {
+    } else if constexpr (MODE == 2) {
         float *d = (float *)out;
 #pragma unroll
-        for (int o = 0; o < CO; ++o) d[p * CO + o] = acc[o] * gelu_grad(zprev[p * CO + o]);
+        for (int o = 0; o < CO; o += 4) {
+            const float4 zp = *(const float4 *)(zprev + p * CO + o);
+            *(float4 *)(d + p * CO + o) = float4{acc[o] * gelu_grad(zp.x), acc[o + 1] * gelu_grad(zp.y),
+                                                 acc[o + 2] * gelu_grad(zp.z), acc[o + 3] * gelu_grad(zp.w)};
+        }
     } else {
         float *d = (float *)out;
 #pragma unroll
@@ -172,6 +206,10 @@ conv32_mfma(const bf16_t *__restrict__ in, int H, int W, const bf16_t *__restric
             const float *__restrict__ bias, float *__restrict__ z_out, bf16_t *__restrict__ a_out,
             int tiles_per_wave)
 {
+    // per-wave output tile [16 px][32 ch] staged in LDS so that the wave
+    // writes 16 contiguous pixel records with 16-byte stores
+    __shared__ bf16_t otile[4][16 * 32];
+    __shared__ float ztile[KEEP_Z ? 4 : 1][KEEP_Z ? 16 * 32 : 1];
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
@@ -213,17 +251,29 @@ conv32_mfma(const bf16_t *__restrict__ in, int H, int W, const bf16_t *__restric
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[tap][hf][1], acc1, 0, 0, 0);
             }
         }
+        const int wv = threadIdx.x >> 6;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const long long p = (long long)h * W + w0 + 4 * g + j;
+            const int px = 4 * g + j;
             const float z0 = acc0[j] + b0, z1 = acc1[j] + b1;
             if (KEEP_Z) {
-                z_out[p * 32 + r] = z0;
-                z_out[p * 32 + 16 + r] = z1;
+                ztile[wv][px * 32 + r] = z0;
+                ztile[wv][px * 32 + 16 + r] = z1;
             }
-            a_out[p * 32 + r] = f2bf(gelu(z0));
-            a_out[p * 32 + 16 + r] = f2bf(gelu(z1));
+            otile[wv][px * 32 + r] = f2bf(gelu(z0));
+            otile[wv][px * 32 + 16 + r] = f2bf(gelu(z1));
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS writes are done
+        const long long p0 = (long long)h * W + w0;
+        // 16 px x 32 ch bf16 = 1 KiB: 64 lanes x 16 B
+        *(uint4 *)(a_out + p0 * 32 + lane * 8) = *(const uint4 *)(&otile[wv][lane * 8]);
+        if (KEEP_Z) {
+            // 16 px x 32 ch f32 = 2 KiB: 64 lanes x 2 x 16 B
+            *(float4 *)(z_out + p0 * 32 + lane * 4) = *(const float4 *)(&ztile[wv][lane * 4]);
+            *(float4 *)(z_out + p0 * 32 + 256 + lane * 4) = *(const float4 *)(&ztile[wv][256 + lane * 4]);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -624,6 +674,22 @@ qx_status qx_net_train_step(qx_net *n, const void *x_dev, const void *t_dev, flo
 }
 
 void *qx_net_stream(qx_net *n) { return n ? (void *)n->stream : nullptr; }
+
+// mean((x - t)^2) over a pair: the loss of an identity corrector
+qx_status qx_net_pair_mse(qx_net *n, const void *x_dev, const void *t_dev, double *mse)
+{
+    if (!n || !x_dev || !t_dev || !mse) return nfail(QX_EINVAL, "null argument");
+    NCHK(hipSetDevice(n->device));
+    const long long ne = n->P * 2;
+    const unsigned gl = grid1(ne);
+    loss_kernel<<<gl, 256, 0, n->stream>>>((const float *)x_dev, (const float *)t_dev, ne, 1.0f, n->d0, n->lpart);
+    NCHK(hipMemcpyAsync(n->h_lpart, n->lpart, sizeof(double) * gl, hipMemcpyDeviceToHost, n->stream));
+    NTRY(nsync(n));
+    double sum = 0.0;
+    for (unsigned b = 0; b < gl; ++b) sum += n->h_lpart[b];
+    *mse = sum / (double)ne;
+    return QX_OK;
+}
 
 qx_status qx_net_set_output_scale(qx_net *n, float s)
 {

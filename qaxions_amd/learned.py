@@ -32,14 +32,15 @@ def init_params(seed=0, last_scale=1e-3):
 def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, log_every=0,
           pref=1.0):
     """Train a corrector on len(seeds) * steps_per_seed pairs.  Returns
-    (net, losses)."""
+    (net, losses, relative) where relative[i] = loss / mean((coarse - fine)^2)
+    of pair i: the fraction of the coarse step's error the network leaves."""
     if net is None:
         net = Corrector(N, N, params=init_params())
     traj = Lattice(2, N, L, "fp32", pref=pref)
     coarse = Lattice(2, N, L, "fp32", pref=pref)
     fine = Lattice(2, N, L, "fp32", pref=pref)
     total = len(seeds) * steps_per_seed
-    losses = []
+    losses, rel = [], []
     it = 0
     scale_set = net.output_scale != 1.0
     for seed in seeds:
@@ -56,14 +57,17 @@ def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, 
                 net.output_scale = float(np.sqrt(np.mean(np.abs(d) ** 2) / 2.0))
                 scale_set = True
             lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(total - 1, 1))
+            base = net.pair_mse(coarse, fine)
             losses.append(net.train_step_lattices(coarse, fine, lr_it))
+            rel.append(losses[-1] / base if base > 0 else 1.0)
             traj.copy_from(fine)
             it += 1
             if log_every and it % log_every == 0:
-                print(f"[train] {it}/{total} loss {np.mean(losses[-log_every:]):.3e}", flush=True)
+                print(f"[train] {it}/{total} loss {np.mean(losses[-log_every:]):.3e} "
+                      f"relative {np.mean(rel[-log_every:]):.3f}", flush=True)
     for x in (traj, coarse, fine):
         x.close()
-    return net, np.array(losses)
+    return net, np.array(losses), np.array(rel)
 
 
 def _rel(a, b):

@@ -91,9 +91,10 @@ def test_trained_corrector_beats_plain_coarse(qx):
     # error stands well above FP32 round-off; at pref = 1 the two agree to
     # round-off and there is nothing to learn
     N, L, dt, k, pref = 64, 10.0, 2e-3, 4, 100.0
-    net, losses = learned.train(N, L, dt, k, seeds=range(1000, 1008), steps_per_seed=50, lr=2e-3,
-                                lr_final=2e-4, pref=pref)
-    assert losses[-50:].mean() < 0.5 * losses[:50].mean()
+    net, losses, relative = learned.train(N, L, dt, k, seeds=range(1000, 1008), steps_per_seed=50,
+                                          lr=2e-3, lr_final=2e-4, pref=pref)
+    # the network removes most of the coarse step's error on pairs it is trained on
+    assert relative[-50:].mean() < 0.5 * relative[:20].mean()
     res = learned.evaluate(net, N, L, dt, k, seed=1, n_coarse=20, check_every=5, mass_tol=1e-3, pref=pref)
     print(res)
     assert res["learned"]["rel_l2_vs_fine"] < 0.7 * res["coarse"]["rel_l2_vs_fine"]
