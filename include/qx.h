@@ -178,12 +178,13 @@ int64_t qx_ens_sites(const qx_ens *ens);
 void *qx_ens_stream(qx_ens *ens);
 
 /* ---- learned coarse-step corrector (no reference counterpart) -------------
- * Periodic 3x3 convolutions 2 -> 32 -> 32 -> 32 -> 2 with GELU (erf form)
- * on (Re psi, Im psi) of an H x W lattice: psi <- psi + s net(psi).  BF16
+ * Periodic 3x3 convolutions 3 -> 32 -> 32 -> 32 -> 2 with GELU (erf form)
+ * on (Re psi, Im psi, a V) of an H x W lattice: psi <- psi + s net(.).  BF16
  * weights and hidden activations, FP32 accumulation; the 32 -> 32 forward
- * layers run on MFMA.  Parameters: 19682 floats, per layer W[co][ci][dy][dx]
+ * layers run on MFMA.  Parameters: 19970 floats, per layer W[co][ci][dy][dx]
  * then b[co], tap (dy, dx) reading (h + dy - 1, w + dx - 1) mod (H, W).
- * x/target device arrays are [H*W][2] float (a complex64 psi array). */
+ * Input device/host arrays x are [H*W][3] float (Re psi, Im psi, a V);
+ * outputs and targets are [H*W][2] float (a complex64 psi array). */
 typedef struct qx_net qx_net;
 int qx_net_param_count(void);
 qx_status qx_net_create(int H, int W, int device, qx_net **out);
@@ -193,18 +194,25 @@ qx_status qx_net_set_params(qx_net *net, const float *theta);  /* resets Adam st
  * size of the correction so the network works with O(1) outputs */
 qx_status qx_net_set_output_scale(qx_net *net, float s);
 float qx_net_output_scale(const qx_net *net);
+/* a: the V channel's scale where the input is built from a lattice
+ * (correct, coarse_steps, train_step_lattices); default 1 */
+qx_status qx_net_set_input_scale(qx_net *net, float a);
+float qx_net_input_scale(const qx_net *net);
 qx_status qx_net_get_params(qx_net *net, float *theta);
-qx_status qx_net_forward(qx_net *net, const void *x_dev, void *y_dev);     /* y = x + net(x) */
+qx_status qx_net_forward(qx_net *net, const void *x_dev, void *y_dev);     /* y = x[:, 0:2] + s net(x) */
 qx_status qx_net_forward_host(qx_net *net, const float *x, float *y);
-/* loss = mean((x + net(x) - target)^2) and its gradient (grad_host nullable) */
+/* loss = mean((x[:, 0:2] + s net(x) - target)^2) and its gradient (grad_host nullable) */
 qx_status qx_net_loss_grad(qx_net *net, const void *x_dev, const void *t_dev, double *loss, float *grad_host);
 qx_status qx_net_loss_grad_host(qx_net *net, const float *x, const float *t, double *loss, float *grad);
 /* Adam on the last gradient (bias-corrected) */
 qx_status qx_net_adam_step(qx_net *net, float lr, float beta1, float beta2, float eps);
 qx_status qx_net_train_step(qx_net *net, const void *x_dev, const void *t_dev, float lr, double *loss);
-/* mean((x - target)^2): the loss an identity corrector would have */
+/* one Adam step on the pair (input from coarse's psi and V, target fine's psi) */
+qx_status qx_net_train_step_lattices(qx_net *net, qx_ctx *coarse, qx_ctx *fine, float lr, double *loss);
+/* mean((x - target)^2) over two psi arrays [H*W][2]: the loss an identity
+ * corrector would have */
 qx_status qx_net_pair_mse(qx_net *net, const void *x_dev, const void *t_dev, double *mse);
-/* psi <- psi + net(psi) on a 2D FP32 context of size H x W (V untouched) */
+/* psi <- psi + s net(psi, a V) on a 2D FP32 context of size H x W (V untouched) */
 qx_status qx_net_correct(qx_net *net, qx_ctx *ctx);
 /* nsteps corrected coarse steps of k*dt each (leapfrog(k dt, 1), correct,
  * update the potential), stream-ordered with no host waits inside a chunk

@@ -2,10 +2,11 @@
 INFRASTRUCTURE: the checker of the HIP forward/backward kernels).
 
 No reference counterpart.  The network: four periodic 3x3 convolutions
-2 -> 32 -> 32 -> 32 -> 2 with GELU (erf form) after the first three; input
-and output are (Re psi, Im psi) channels; the corrected field is
-psi_coarse + net(psi_coarse).  Loss: mean over pixels and the two channels
-of (psi_coarse + net(psi_coarse) - psi_fine)^2.
+3 -> 32 -> 32 -> 32 -> 2 with GELU (erf form) after the first three; the
+input channels are (Re psi, Im psi, a V), the output channels a correction
+to (Re psi, Im psi); the corrected field is psi_coarse + s net(x).  Loss:
+mean over pixels and the two psi channels of
+(psi_coarse + s net(x) - psi_fine)^2.
 
 Layouts: activations [H][W][C]; weights W[co][ci][dy][dx] with tap
 (dy, dx) reading x[(h + dy - 1) mod H, (w + dx - 1) mod W]; parameters
@@ -19,7 +20,7 @@ import math
 import numpy as np
 from scipy.special import erf
 
-CHANNELS = (2, 32, 32, 32, 2)
+CHANNELS = (3, 32, 32, 32, 2)
 
 
 def param_shapes():
@@ -128,12 +129,12 @@ def forward(theta, x, keep=False, bf16=False):
 
 
 def loss_and_grad(theta, x, target, bf16=False, scale=1.0):
-    """loss = mean((x + scale net(x) - target)^2); gradient w.r.t. theta
+    """loss = mean((x[..., :2] + scale net(x) - target)^2); gradient w.r.t. theta
     (with bf16=True the rounding is treated as the identity in the backward
     pass, as the device kernels do)"""
     layers = _layers(theta, bf16)
     out, cache = forward(theta, x, keep=True, bf16=bf16)
-    r = x + scale * out - target
+    r = np.asarray(x, np.float64)[..., :2] + scale * out - target
     n = r.size
     loss = float((r * r).sum() / n)
     d = 2.0 * scale * r / n
@@ -147,8 +148,10 @@ def loss_and_grad(theta, x, target, bf16=False, scale=1.0):
     return loss, flatten(grads[::-1])
 
 
-def psi_to_channels(psi):
-    return np.stack([psi.real, psi.imag], axis=-1)
+def psi_to_channels(psi, V=None, a=1.0):
+    """network input [H, W, 3] from psi and V (V = 0 when absent)"""
+    v = np.zeros(psi.shape) if V is None else a * np.asarray(V, np.float64)
+    return np.stack([psi.real, psi.imag, v], axis=-1)
 
 
 def channels_to_psi(x):

@@ -104,6 +104,9 @@ SIGNATURES = {
     "qx_net_get_params": (C.c_int, [C.c_void_p, _fp]),
     "qx_net_set_output_scale": (C.c_int, [C.c_void_p, C.c_float]),
     "qx_net_output_scale": (C.c_float, [C.c_void_p]),
+    "qx_net_set_input_scale": (C.c_int, [C.c_void_p, C.c_float]),
+    "qx_net_input_scale": (C.c_float, [C.c_void_p]),
+    "qx_net_train_step_lattices": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, _dp]),
     "qx_net_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "qx_net_forward_host": (C.c_int, [C.c_void_p, _fp, _fp]),
     "qx_net_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _fp]),
@@ -473,7 +476,8 @@ def _check_net(status):
 
 
 class Corrector:
-    """Learned coarse-step corrector psi <- psi + net(psi) on H x W lattices."""
+    """Learned coarse-step corrector psi <- psi + s net(psi, a V) on H x W
+    lattices (s: output scale, a: input scale of the potential channel)."""
 
     def __init__(self, H, W, device=0, params=None):
         self.H, self.W = int(H), int(W)
@@ -515,16 +519,27 @@ class Corrector:
     def output_scale(self, s):
         _check_net(lib().qx_net_set_output_scale(self._h, float(s)))
 
+    @property
+    def input_scale(self):
+        return float(lib().qx_net_input_scale(self._h))
+
+    @input_scale.setter
+    def input_scale(self, a):
+        _check_net(lib().qx_net_set_input_scale(self._h, float(a)))
+
     def forward(self, x):
-        """x: [H, W, 2] float array -> x + s net(x)"""
+        """x: [H, W, 3] float array (Re psi, Im psi, a V) -> x[..., :2] + s net(x)"""
         x = np.ascontiguousarray(x, np.float32)
-        y = np.empty_like(x)
+        assert x.shape == (self.H, self.W, 3)
+        y = np.empty((self.H, self.W, 2), np.float32)
         _check_net(lib().qx_net_forward_host(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp)))
         return y
 
     def loss_grad(self, x, t):
+        """x: [H, W, 3] input, t: [H, W, 2] target -> (loss, gradient)"""
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
+        assert x.shape == (self.H, self.W, 3) and t.shape == (self.H, self.W, 2)
         g = np.zeros(self.n_params, np.float32)
         loss = C.c_double()
         _check_net(lib().qx_net_loss_grad_host(self._h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp),
@@ -532,12 +547,11 @@ class Corrector:
         return loss.value, g
 
     def train_step_lattices(self, coarse, fine, lr):
-        """one Adam step on the pair (coarse.psi -> fine.psi), both FP32 Lattices"""
-        pc, pf = C.c_void_p(), C.c_void_p()
-        _check(lib().qx_device_state(coarse.handle, C.byref(pc), None))
-        _check(lib().qx_device_state(fine.handle, C.byref(pf), None))
+        """one Adam step on the pair ((coarse.psi, coarse.V) -> fine.psi), both
+        FP32 Lattices"""
         loss = C.c_double()
-        _check_net(lib().qx_net_train_step(self._h, pc, pf, float(lr), C.byref(loss)))
+        _check_net(lib().qx_net_train_step_lattices(self._h, coarse.handle, fine.handle, float(lr),
+                                                    C.byref(loss)))
         return loss.value
 
     def pair_mse(self, coarse, fine):

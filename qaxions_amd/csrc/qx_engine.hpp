@@ -13,18 +13,24 @@ namespace qx {
 // ---------------------------------------------------------------------------
 struct Engine {
     virtual ~Engine() = default;
-    virtual void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) = 0;
+    // out: optional destination with swapped outer indices (contig_fft)
+    virtual void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s,
+                        void *out = nullptr, long long rm_m = 1, long long rm_a = 1) = 0;
     virtual void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                          hipStream_t s) = 0;
+    // out-of-place column pass between data and its column tiles (see
+    // strided_fft): forward, op < 0, data columns -> tiles; inverse, op > 0,
+    // tiles -> data columns.  Tiles are tile_cols() wide; inner must be a
+    // multiple of it
+    virtual void strided_tiles(void *data, void *tiles, long long outer, long long inner, int op,
+                               hipStream_t s) = 0;
+    virtual int tile_cols() const = 0;
     // rows of `rpm` rows per member (rpm = rows for a single lattice)
-    // psit: the k-major copy of psi's rows (R2C_INV_DENSITY_T input,
-    // C2R_FWD_T output); unused otherwise
     virtual void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows,
-                          long long rpm, double pref, double rho_bar, hipStream_t s,
-                          const void *psit = nullptr) = 0;
+                          long long rpm, double pref, double rho_bar, hipStream_t s) = 0;
     virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm,
                           double kick_dt, const double *kick_member, int flags,
-                          unsigned long long *vmax, hipStream_t s, void *psit = nullptr) = 0;
+                          unsigned long long *vmax, hipStream_t s) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
     // tile variant per strided pass kind (0 = full tile, 1 = half tile),
     // indexed by [op kind][stride class]; set by qx_tune
@@ -53,16 +59,35 @@ template <int N, typename R> struct EngineT final : Engine {
         return (long long)cus * per;
     }
 
-    void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s) override
+    void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s, void *out,
+                long long rm_m, long long rm_a) override
     {
         const unsigned blocks = (unsigned)((nlines + G::LPB - 1) / G::LPB);
-        C *d = (C *)data;
+        C *d = (C *)data, *o = (C *)out;
         const C *t = (const C *)tw;
-        if (op < 0) contig_fft<N, R, OP_FWD><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
-        else if (op > 0) contig_fft<N, R, OP_INV><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
-        else if (m.kind == MUL_DRIFT) contig_fft<N, R, OP_DRIFT><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
-        else contig_fft<N, R, OP_POISSON><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t);
+        if (op < 0) contig_fft<N, R, OP_FWD><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
+        else if (op > 0) contig_fft<N, R, OP_INV><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
+        else contig_fft<N, R, OP_POISSON><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
     }
+
+    // the tile width is the full column tile, whatever the variant of the
+    // plain passes: the tile layout must not depend on tuning
+    void strided_tiles(void *data, void *tiles, long long outer, long long inner, int op, hipStream_t s) override
+    {
+        constexpr int TC = G::TC;
+        const long long ntiles = inner / TC;
+        const unsigned blocks = (unsigned)(outer * ntiles);
+        const int threads = TC * P::TPL;
+        MulArgs none{};
+        none.member_div = 1;
+        C *d = (C *)data, *l = (C *)tiles;
+        const C *t = (const C *)tw;
+        if (op < 0)
+            strided_fft<N, R, OP_FWD, TC, XM_STORE_TILES><<<blocks, threads, 0, s>>>(d, inner, ntiles, none, t, l, outer);
+        else
+            strided_fft<N, R, OP_INV, TC, XM_LOAD_TILES><<<blocks, threads, 0, s>>>(d, inner, ntiles, none, t, l, outer);
+    }
+    int tile_cols() const override { return G::TC; }
 
     template <int TC>
     void strided_tc(void *data, long long outer, long long inner, int op, const MulArgs &m, hipStream_t s)
@@ -72,10 +97,10 @@ template <int N, typename R> struct EngineT final : Engine {
         const int threads = TC * P::TPL;
         C *d = (C *)data;
         const C *t = (const C *)tw;
-        if (op < 0) strided_fft<N, R, OP_FWD, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
-        else if (op > 0) strided_fft<N, R, OP_INV, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
-        else if (m.kind == MUL_DRIFT) strided_fft<N, R, OP_DRIFT, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
-        else strided_fft<N, R, OP_POISSON, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t);
+        if (op < 0) strided_fft<N, R, OP_FWD, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        else if (op > 0) strided_fft<N, R, OP_INV, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        else if (m.kind == MUL_DRIFT) strided_fft<N, R, OP_DRIFT, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        else strided_fft<N, R, OP_POISSON, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
     }
     void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                  hipStream_t s) override
@@ -84,7 +109,7 @@ template <int N, typename R> struct EngineT final : Engine {
         else strided_tc<G::TC>(data, outer, inner, op, m, s);
     }
     void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
-                  double pref, double rho_bar, hipStream_t s, const void *psit) override
+                  double pref, double rho_bar, hipStream_t s) override
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         C *p = (C *)psi;
@@ -92,22 +117,19 @@ template <int N, typename R> struct EngineT final : Engine {
         C *o = (C *)vt;
         const C *t = (const C *)tw;
         if (mode == R2C_REAL)
-            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
+            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else if (mode == R2C_DENSITY)
-            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
-        else if (mode == R2C_INV_DENSITY)
-            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
+            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else
-            qx::r2c_rows<N, R, R2C_INV_DENSITY_T><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, (const C *)psit);
+            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
-                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s,
-                  void *psit) override
+                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s) override
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
                                                              (R)kick_dt, kick_member, flags, vmax,
-                                                             (const C *)tw, (C *)psit);
+                                                             (const C *)tw);
     }
 };
 

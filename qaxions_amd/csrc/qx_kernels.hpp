@@ -44,21 +44,16 @@ template <int N, typename R> struct Geom {
     // row kernels: pairs of rows per block
     static constexpr int NP0 = (256 / P::TPL) > 8 ? (256 / P::TPL) : 8;
     static constexpr int NP1 = (NP0 * P::TPL > 1024) ? (1024 / P::TPL) : NP0;
-    // LDS budget of ~80 KiB: per pair max(exchange line, two staging rows)
-    // per pair: the exchange line, or two rows of k-major staging (2N slots)
-    static constexpr int PAIR_SLOTS = P::PADDED > 2 * N ? P::PADDED : 2 * N;
+    // LDS budget of ~80 KiB: per pair max(exchange line, two half-spectrum
+    // staging rows)
+    static constexpr int PAIR_SLOTS = P::PADDED > N + 2 ? P::PADDED : N + 2;
     static constexpr int NP2 = (81920 - N * (int)sizeof(C)) / (PAIR_SLOTS * (int)sizeof(C));
     static constexpr int NP = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
     static constexpr int TL = 2 * NP;                 // rows per block
     static constexpr int ROW_THREADS = NP * P::TPL;
     static constexpr int XCH = NP * P::PADDED;        // exchange slots
     static constexpr int STG = (N / 2 + 1) * TL;      // staging slots (half spectrum)
-    // staging slots for full rows k-major, when they fit beside the twiddles
-    // (else the k-major rows are read and written element by element)
-    static constexpr bool STG_T_LDS = (long long)(N * TL + N) * (long long)sizeof(C) + 1024 <= 163840;
-    static constexpr int STG_T = STG_T_LDS ? N * TL : 0;
-    static constexpr int ROW_LDS0 = XCH > STG ? XCH : STG;
-    static constexpr int ROW_LDS = ROW_LDS0 > STG_T ? ROW_LDS0 : STG_T;
+    static constexpr int ROW_LDS = XCH > STG ? XCH : STG;
 };
 
 // pointwise factor in fused passes
@@ -74,8 +69,8 @@ enum {
     GEO_PSI2 = 1,  // psi 2D, axis 0 strided, c = k1
     GEO_VT3 = 2,   // vt 3D, axis i0 strided inside plane o = k2, c = k1
     GEO_VT2 = 3,   // vt 2D, contiguous lines along i0, line index = k1
-    GEO_PSIT3 = 4, // psi k2-major [k2][i0][k1]: axis i0 strided in plane o = k2, c = k1
-    GEO_PSIT2 = 5, // psi k1-major [k1][i0]: contiguous lines along i0, line = k1
+    GEO_PSIT3 = 4, // 3D psi in column tiles [k2 / TT][i0][k1][k2 % TT] (TT = tcols):
+                   // axis i0 strided, o = k2 / TT, c = k1 * TT + k2 % TT
 };
 
 struct MulArgs {
@@ -93,6 +88,7 @@ struct MulArgs {
     // line index) / member_div; null for a single lattice
     const double *member_par;
     long long member_div;
+    int tcols;       // GEO_PSIT3: columns per tile
 };
 
 // the multiply's constants for one member of an ensemble
@@ -203,12 +199,16 @@ __device__ __forceinline__ void load_tw(C *tw, const C *twg)
 }
 
 // ---------------------------------------------------------------------------
-// contiguous lines: data[line*N + pos], LPB lines per block
+// contiguous lines: data[line*N + pos], LPB lines per block.  With `out`, the
+// result of line l = a*rm_m + b (b < rm_m) goes to out line b*rm_a + a: the
+// two outer indices of a 3-index array swap places (rm_a = count of a), so a
+// block's LPB lines land in LPB whole lines of the other layout
 // ---------------------------------------------------------------------------
-template <int N, typename R, int OP>  // OP_FWD, OP_INV, OP_POISSON (2D vt lines), OP_DRIFT (2D psi, k1-major)
+template <int N, typename R, int OP>  // OP_FWD, OP_INV or OP_POISSON (2D vt lines)
 __global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
 contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul,
-           const typename cplx<R>::T *__restrict__ twg)
+           const typename cplx<R>::T *__restrict__ twg, typename cplx<R>::T *__restrict__ out,
+           long long rm_m, long long rm_a)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -229,7 +229,8 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
     } else if constexpr (OP == OP_INV) {
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
-    } else if constexpr (OP == OP_POISSON) {
+    } else {
+        static_assert(OP == OP_POISSON, "contiguous passes: fwd, inv or the 2D Poisson solve");
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
         // GEO_VT2: position = k0 (folded), line = member * (N/2+1) + k1
         const int k1 = (int)(line % mul.member_div);
@@ -240,18 +241,11 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
             v[q] = poisson_factor<R>(ma, n0 * n0 + k1 * k1, v[q]);
         }
         fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
-    } else {  // OP_DRIFT, GEO_PSIT2: position = i0, line = member * N + k1
-        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
-        const long long ln = ok ? line : 0;
-        const MulArgs ma = member_args(mul, ln / mul.member_div);
-        const double2 cf = ma.ptab[(int)(ln % mul.member_div)];
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = drift_factor(ma.ptab[t + q * P::TPL], cf, ma.inv_s, v[q]);
-        fft_line<N, +1, LdsContig>(v, t, mine, 1, tw);
     }
     if (ok) {
+        C *dst = out ? out + ((line % rm_m) * rm_a + line / rm_m) * N : src;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) src[t + q * P::TPL] = v[q];
+        for (int q = 0; q < P::E; ++q) dst[t + q * P::TPL] = v[q];
     }
 }
 
@@ -261,11 +255,24 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
 // (Geom::TC or half of it): narrower tiles use half the LDS, so twice as
 // many blocks share a CU and overlap their loads with each other's
 // transforms; the variant is picked per pass by measurement.
+//
+// Out-of-place tile modes (3D psi, whose rows have been transformed along
+// the last axis): a block's N x TC tile of columns (col0..col0+TC-1 of outer
+// index o) lives contiguously in `tiles` at ((col0/TC)*outer + o)*N*TC, as
+// [pos][col % TC].  XM_STORE_TILES transforms the columns of data and writes
+// the tile; XM_LOAD_TILES reads the tile, transforms it and writes columns of
+// data.  With data = psi[i0][i1][k2] (o = i0, col = k2) the tiles form
+// [k2/TC][i0][k1][k2%TC]: the i0 axis is a column pass of stride N*TC over
+// them (GEO_PSIT3), 128-byte segments in 16 pages per block, where psi's own
+// i0 axis (stride N^2) would take one page per element.
 // ---------------------------------------------------------------------------
-template <int N, typename R, int OP, int TC>  // OP_FWD, OP_INV, OP_DRIFT or OP_POISSON
+enum { XM_NONE = 0, XM_STORE_TILES = 1, XM_LOAD_TILES = 2 };
+
+template <int N, typename R, int OP, int TC, int XM = XM_NONE>  // OP_FWD, OP_INV, OP_DRIFT or OP_POISSON
 __global__ void __launch_bounds__((TC * Plan<N>::TPL))
 strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
-            MulArgs mul, const typename cplx<R>::T *__restrict__ twg)
+            MulArgs mul, const typename cplx<R>::T *__restrict__ twg,
+            typename cplx<R>::T *__restrict__ tiles, long long outer)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -287,9 +294,15 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         ptab = ptab_lds;
     }
     C *base = data + o * (long long)N * inner + col;
+    C *tile = tiles + ((b % ntiles) * outer + o) * (long long)N * TC + c;
     C v[P::E];
+    if constexpr (XM == XM_LOAD_TILES) {
 #pragma unroll
-    for (int q = 0; q < P::E; ++q) v[q] = ok ? base[(long long)(t + q * P::TPL) * inner] : C{0, 0};
+        for (int q = 0; q < P::E; ++q) v[q] = ok ? tile[(t + q * P::TPL) * TC] : C{0, 0};
+    } else {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = ok ? base[(long long)(t + q * P::TPL) * inner] : C{0, 0};
+    }
     __syncthreads();
     if constexpr (OP == OP_FWD) {
         fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
@@ -301,7 +314,8 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         if constexpr (OP == OP_DRIFT) {
             double2 cf;
             if (mul.geo == GEO_PSI3) cf = zmul(ptab[cc / N], ptab[cc % N]);
-            else if (mul.geo == GEO_PSIT3) cf = zmul(ptab[cc], ptab[(int)o]);
+            else if (mul.geo == GEO_PSIT3)
+                cf = zmul(ptab[cc / mul.tcols], ptab[(int)o * mul.tcols + cc % mul.tcols]);
             else cf = ptab[cc];
 #pragma unroll
             for (int q = 0; q < P::E; ++q) v[q] = drift_factor(ptab[t + q * P::TPL], cf, ma.inv_s, v[q]);
@@ -316,7 +330,12 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         }
         fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
     }
-    if (ok) {
+    if constexpr (XM == XM_STORE_TILES) {
+        if (ok) {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) tile[(t + q * P::TPL) * TC] = v[q];
+        }
+    } else if (ok) {
 #pragma unroll
         for (int q = 0; q < P::E; ++q) base[(long long)(t + q * P::TPL) * inner] = v[q];
     }
@@ -330,15 +349,13 @@ enum {
     R2C_DENSITY = 1,   // input: psi rows, rho = pref (|psi|^2 - rho_bar)
     R2C_INV_DENSITY = 2,  // input: psi rows transformed along the last axis;
                           // inverse-transform, store psi, then as DENSITY
-    R2C_INV_DENSITY_T = 3,  // as INV_DENSITY, the transformed rows read k-major
-                            // from psit[k * rows + row] (per member)
 };
 
 template <int N, typename R, int MODE>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
          typename cplx<R>::T *__restrict__ vt, long long rows, long long rpm, R pref, R rho_bar,
-         const typename cplx<R>::T *__restrict__ twg, const typename cplx<R>::T *__restrict__ psit)
+         const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -363,37 +380,14 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
         __syncthreads();
     } else {
         C a[P::E], b[P::E];
-        if constexpr (MODE == R2C_INV_DENSITY_T && !G::STG_T_LDS) {
 #pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                a[q] = oka ? psit[vt_index(ra, pos, rpm, N)] : C{0, 0};
-                b[q] = okb ? psit[vt_index(rb, pos, rpm, N)] : C{0, 0};
-            }
-        } else if constexpr (MODE == R2C_INV_DENSITY_T) {
-            // k-major rows: stage [k][l] for the block's TL rows, then gather
-            C *stg = lds;
-            for (int i = threadIdx.x; i < N * TL; i += blockDim.x) {
-                const int k = i / TL, l = i % TL;
-                stg[i] = (r0 + l < rows) ? psit[vt_index(r0 + l, k, rpm, N)] : C{0, 0};
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                a[q] = stg[pos * TL + 2 * p];
-                b[q] = stg[pos * TL + 2 * p + 1];
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-                b[q] = okb ? psi[rb * N + pos] : C{0, 0};
-            }
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
         }
         __syncthreads();
-        if (MODE == R2C_INV_DENSITY || MODE == R2C_INV_DENSITY_T) {
+        if (MODE == R2C_INV_DENSITY) {
             fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
             fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
 #pragma unroll
@@ -455,7 +449,6 @@ enum {
     C2R_VMAX = 4,       // fold max|V| into *vmax_bits
     C2R_KICK = 8,       // psi <- psi exp(-i kick_dt V)
     C2R_FWD = 16,       // forward-transform psi along the last axis
-    C2R_FWD_T = 32,     // with C2R_FWD: store the transformed rows k-major to psit
 };
 
 template <int N, typename R>
@@ -463,8 +456,7 @@ __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
          const double *__restrict__ kick_member, int flags,
-         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg,
-         typename cplx<R>::T *__restrict__ psit)
+         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -479,12 +471,34 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     const long long ra = r0 + 2 * p, rb = ra + 1;
     const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
+    // psi's rows first: their loads overlap the potential's transform
+    const bool upd = (flags & (C2R_KICK | C2R_FWD)) != 0;
+    C a[P::E], b[P::E];
+    if (upd) {
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+        }
+    }
     R va[P::E], vb[P::E];
     if (flags & C2R_SRC_VT) {
+        // stage [k][l] of the block's TL rows: all loads issued before the
+        // first LDS store
         C *stg = lds;
-        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+        constexpr int NS = (H1 * TL + G::ROW_THREADS - 1) / G::ROW_THREADS;
+        C tmp[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int i = threadIdx.x + j * G::ROW_THREADS;
             const int k = i / TL, l = i % TL;
-            stg[i] = (r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
+            tmp[j] = (i < H1 * TL && r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int i = threadIdx.x + j * G::ROW_THREADS;
+            if (i < H1 * TL) stg[i] = tmp[j];
         }
         __syncthreads();
         C z[P::E];
@@ -534,14 +548,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(m));
         }
     }
-    if (flags & (C2R_KICK | C2R_FWD)) {
-        C a[P::E], b[P::E];
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
-        }
+    if (upd) {
         if (flags & C2R_KICK) {
             const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
             const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
@@ -559,35 +566,11 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
             fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
         }
-        if ((flags & C2R_FWD_T) && !G::STG_T_LDS) {
 #pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                if (oka) psit[vt_index(ra, pos, rpm, N)] = a[q];
-                if (okb) psit[vt_index(rb, pos, rpm, N)] = b[q];
-            }
-        } else if (G::STG_T_LDS && (flags & C2R_FWD_T)) {
-            // k-major store: stage [k][l], then TL consecutive rows per k
-            __syncthreads();
-            C *stg = lds;
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                stg[pos * TL + 2 * p] = a[q];
-                stg[pos * TL + 2 * p + 1] = b[q];
-            }
-            __syncthreads();
-            for (int i = threadIdx.x; i < N * TL; i += blockDim.x) {
-                const int k = i / TL, l = i % TL;
-                if (r0 + l < rows) psit[vt_index(r0 + l, k, rpm, N)] = stg[i];
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                if (oka) psi[ra * N + pos] = a[q];
-                if (okb) psi[rb * N + pos] = b[q];
-            }
+        for (int q = 0; q < P::E; ++q) {
+            const int pos = t + q * P::TPL;
+            if (oka) psi[ra * N + pos] = a[q];
+            if (okb) psi[rb * N + pos] = b[q];
         }
     }
 }

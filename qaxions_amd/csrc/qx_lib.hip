@@ -4,18 +4,19 @@
 // and Field's operators (src/field.cxx:93-402); the fused schedule of
 // qx_leapfrog merges neighbouring operators into shared HBM passes:
 //
-//   3D step:  c2r_rows[vt -> V, kick, fwd axis 2, store k2-major psit]
-//                                                       (K + start of D)
-//             contig fwd axis 1 on psit = [k2][i0][i1]
-//             strided axis 0: fwd, e^{-i dt k^2/2}/S, inv (stride N)  (D)
-//             contig inv axis 1
-//             r2c_rows[gather psit rows, inv axis 2, store psi, density,
-//                      r2c]                              (end of D, P)
-//             contig fwd / strided Poisson / contig inv on vt (P)
-//   2D step:  c2r_rows[.., store k1-major psit] / contig drift on psit /
-//             r2c_rows[gather psit] / contig Poisson.
-// The k-major copy psit turns the leading axis' transform, which would stride
-// N^2 elements in psi's own layout, into contiguous or stride-N lines.
+//   3D step:  c2r_rows[vt -> V, kick, fwd axis 2]          (K + start of D)
+//             strided fwd axis 1, psi columns -> column tiles psit
+//             strided axis 0 on psit: fwd, e^{-i dt k^2/2}/S, inv   (D)
+//             strided inv axis 1, psit -> psi columns
+//             r2c_rows[inv axis 2, store psi, density, r2c]  (end of D, P)
+//             contig fwd axis 1 on vt -> k-major vtk, strided Poisson on
+//             vtk, contig inv axis 1 vtk -> vt                         (P)
+//   2D step:  c2r_rows / strided drift / r2c_rows / contig Poisson.
+// psit = [k2/TC][i0][k1][k2%TC] (TC columns per tile) makes the leading
+// axis a column pass of stride N*TC that touches few pages per block; in
+// psi's own layout it would stride N^2 elements.  vt is plane-major,
+// [i0][k2][i1], so each row block of the real<->complex passes reads and
+// writes one plane.
 #include "../../include/qx.h"
 #include "qx_aux.hpp"
 #include "qx_engine.hpp"
@@ -113,7 +114,8 @@ struct qx_ctx {
     long long S = 0, rows = 0, H1 = 0, H = 0;
     size_t csz = 0, rsz = 0;
     void *psi = nullptr, *V = nullptr, *vt = nullptr;
-    void *psit = nullptr;    // k-major copy of psi inside a fused step (lazy)
+    void *psit = nullptr;    // 3D: psi's column tiles inside a drift (lazy)
+    void *vtk = nullptr;     // 3D: k-major copy of vt inside a Poisson solve (lazy)
     void *aux_c = nullptr;   // S complex scratch (lazy)
     void *aux_vt = nullptr;  // H complex scratch (lazy)
     void *aux_r = nullptr;   // S real scratch (lazy)
@@ -244,9 +246,12 @@ void fill_phase_table(double2 *T, int N, double L, double dt)
     }
 }
 
-// kmajor: the factor applied on the k-major copy psit (fused step) rather
-// than on psi in its own layout (qx_drift)
-qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool kmajor = false)
+// 3D lattices whose leading-axis drift runs on psi's column tiles
+bool tiled(const qx_ctx *c) { return c->dim == 3 && c->N >= c->eng->tile_cols(); }
+
+// tiles: the factor applied on the column tiles psit (tiled(c)) rather than
+// on psi in its own layout
+qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool tiles = false)
 {
     if (!c->ptab) TRY(ensure(c, (void **)&c->ptab, sizeof(double2) * c->N));
     if (!(c->ptab_dt == dt)) {
@@ -258,11 +263,12 @@ qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool kmajor = false)
     }
     *m = MulArgs{};
     m->kind = MUL_DRIFT;
-    m->geo = kmajor ? (c->dim == 3 ? GEO_PSIT3 : GEO_PSIT2) : (c->dim == 3 ? GEO_PSI3 : GEO_PSI2);
+    m->geo = tiles ? GEO_PSIT3 : (c->dim == 3 ? GEO_PSI3 : GEO_PSI2);
+    m->tcols = tiles ? c->eng->tile_cols() : 1;
     m->ptab = c->ptab;
     m->ptab_stride = 0;
     m->inv_s = 1.0 / (double)c->S;
-    m->member_div = (kmajor && c->dim == 2) ? c->N : 1;
+    m->member_div = 1;
     return QX_OK;
 }
 MulArgs poisson_args(const qx_ctx *c)
@@ -297,7 +303,16 @@ qx_status drift_core(qx_ctx *c, double dt)
     const int N = c->N;
     MulArgs m;
     TRY(drift_args(c, dt, &m));
-    if (c->dim == 3) {
+    if (tiled(c)) {
+        // axis 1 into psi's column tiles, axis 0 on the tiles, axis 1 back
+        const long long TC = c->eng->tile_cols();
+        TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+        MulArgs mk;
+        TRY(drift_args(c, dt, &mk, true));
+        c->eng->strided_tiles(c->psi, c->psit, N, N, -1, c->stream);
+        c->eng->strided(c->psit, N / TC, N * TC, 0, mk, c->stream);
+        c->eng->strided_tiles(c->psi, c->psit, N, N, +1, c->stream);
+    } else if (c->dim == 3) {
         c->eng->strided(c->psi, N, N, -1, none, c->stream);
         c->eng->strided(c->psi, 1, (long long)N * N, 0, m, c->stream);
         c->eng->strided(c->psi, N, N, +1, none, c->stream);
@@ -328,18 +343,23 @@ void vt_inverse(qx_ctx *c, void *vt)
         c->eng->contig(vt, c->H1, +1, none, c->stream);
     }
 }
-void vt_poisson(qx_ctx *c, void *vt)
+// Poisson solve on the potential's half spectrum in the plane-major layout
+// of c->vt (3D: [i0][k2][i1], 2D: [k1][i0]); 3D goes through the k-major
+// scratch vtk = [k2][i0][k1] and back
+qx_status vt_poisson(qx_ctx *c, void *vt)
 {
     MulArgs none{};
     const int N = c->N;
     const MulArgs m = poisson_args(c);
     if (c->dim == 3) {
-        c->eng->contig(vt, c->H1 * N, -1, none, c->stream);
-        c->eng->strided(vt, c->H1, N, 0, m, c->stream);
-        c->eng->contig(vt, c->H1 * N, +1, none, c->stream);
+        TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
+        c->eng->contig(vt, c->H1 * N, -1, none, c->stream, c->vtk, c->H1, N);
+        c->eng->strided(c->vtk, c->H1, N, 0, m, c->stream);
+        c->eng->contig(c->vtk, c->H1 * N, +1, none, c->stream, vt, N, c->H1);
     } else {
         c->eng->contig(vt, c->H1, 0, m, c->stream);
     }
+    return QX_OK;
 }
 
 double rho_bar_eff(const qx_ctx *c) { return c->ph.rho_bar; }
@@ -364,10 +384,10 @@ qx_status read_vmax(qx_ctx *c)
 qx_status potential_from_psi(qx_ctx *c)
 {
     TRY(reset_vmax(c));
-    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->rows, c->ph.pref, rho_bar_eff(c),
+    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->N, c->ph.pref, rho_bar_eff(c),
                      c->stream);
-    vt_poisson(c, c->vt);
-    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V | C2R_VMAX,
+    TRY(vt_poisson(c, c->vt));
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->N, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V | C2R_VMAX,
                      c->vmax_bits, c->stream);
     return read_vmax(c);
 }
@@ -403,31 +423,41 @@ bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
 int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 8 : 4; }
 
 // pass p of one fused KDK step acting on psi (real space, row layout), its
-// k-major copy psit and vt; see the file header.  dm is drift_args(kmajor)
-void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, const MulArgs &dm,
+// column tiles psit, vt (plane-major) and vt's k-major scratch vtk; see the
+// file header.  dm is drift_args(c, ds, &dm, tiled(c))
+void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, void *vtk, const MulArgs &dm,
                const MulArgs &pm)
 {
     MulArgs none{};
     const int N = c->N;
-    const int kick = C2R_SRC_VT | C2R_KICK | C2R_FWD | C2R_FWD_T;
+    const int kick = C2R_SRC_VT | C2R_KICK | C2R_FWD;
     if (c->dim == 3) {
-        // psit = [k2][i0][i1]: the i1 passes are contiguous, the i0 pass N-strided
+        const bool t = tiled(c);
+        const long long TC = c->eng->tile_cols();
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, psit); break;
-        case 1: c->eng->contig(psit, c->rows, -1, none, c->stream); break;
-        case 2: c->eng->strided(psit, N, N, 0, dm, c->stream); break;
-        case 3: c->eng->contig(psit, c->rows, +1, none, c->stream); break;
-        case 4: c->eng->r2c_rows(R2C_INV_DENSITY_T, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream, psit); break;
-        case 5: c->eng->contig(vt, c->H1 * N, -1, none, c->stream); break;
-        case 6: c->eng->strided(vt, c->H1, N, 0, pm, c->stream); break;
-        case 7: c->eng->contig(vt, c->H1 * N, +1, none, c->stream); break;
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, N, ds, nullptr, kick, nullptr, c->stream); break;
+        case 1:
+            if (t) c->eng->strided_tiles(psi, psit, N, N, -1, c->stream);
+            else c->eng->strided(psi, N, N, -1, none, c->stream);
+            break;
+        case 2:
+            if (t) c->eng->strided(psit, N / TC, N * TC, 0, dm, c->stream);
+            else c->eng->strided(psi, 1, (long long)N * N, 0, dm, c->stream);
+            break;
+        case 3:
+            if (t) c->eng->strided_tiles(psi, psit, N, N, +1, c->stream);
+            else c->eng->strided(psi, N, N, +1, none, c->stream);
+            break;
+        case 4: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, N, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 5: c->eng->contig(vt, c->H1 * N, -1, none, c->stream, vtk, c->H1, N); break;
+        case 6: c->eng->strided(vtk, c->H1, N, 0, pm, c->stream); break;
+        case 7: c->eng->contig(vtk, c->H1 * N, +1, none, c->stream, vt, N, c->H1); break;
         }
     } else {
-        // psit = [k1][i0]: the i0 drift pass is contiguous
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, psit); break;
-        case 1: c->eng->contig(psit, c->rows, 0, dm, c->stream); break;
-        case 2: c->eng->r2c_rows(R2C_INV_DENSITY_T, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream, psit); break;
+        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream); break;
+        case 1: c->eng->strided(psi, 1, N, 0, dm, c->stream); break;
+        case 2: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
     }
@@ -533,7 +563,7 @@ qx_status qx_destroy(qx_ctx *c)
     if (!c) return QX_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->psi, c->V, c->vt, c->psit, c->aux_c, c->aux_vt, c->aux_r, c->stage,
+    void *bufs[] = {c->psi, c->V, c->vt, c->psit, c->vtk, c->aux_c, c->aux_vt, c->aux_r, c->stage,
                     c->partials, c->vmax_bits, c->spec_P, c->spec_cnt, c->ptab,
                     c->eng ? c->eng->tw : nullptr};
     for (void *b : bufs)
@@ -771,7 +801,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, co
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < GRAPH_STEPS; ++i)
-                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, dm, pm);
+                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, dm, pm);
             HIPCHK(hipStreamEndCapture(c->stream, &g));
             const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -782,7 +812,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, co
         for (; done + GRAPH_STEPS <= n; done += GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     }
     for (; done < n; ++done)
-        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, dm, pm);
+        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, dm, pm);
     return QX_OK;
 }
 
@@ -798,17 +828,18 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
     }
     const MulArgs pm = poisson_args(c);
     MulArgs dm;
-    TRY(drift_args(c, ds, &dm, true));
-    TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+    TRY(drift_args(c, ds, &dm, tiled(c)));
+    if (tiled(c)) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+    if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
     const int np = step_pass_count(c);
     // the opening kick reads the stored V; later kicks the potential's spectrum
-    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->rows, k_first, nullptr, C2R_KICK | C2R_FWD | C2R_FWD_T,
-                     nullptr, c->stream, c->psit);
-    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, dm, pm);
+    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->N, k_first, nullptr, C2R_KICK | C2R_FWD, nullptr,
+                     c->stream);
+    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, dm, pm);
     TRY(steady_steps(c, nsteps - 1, ds, dm, pm));
     // last step's c2r: materialise V and max|V|, closing half kick
     if (wait) TRY(reset_vmax(c));
-    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.5 * ds, nullptr,
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->N, 0.5 * ds, nullptr,
                      C2R_SRC_VT | C2R_STORE_V | (wait ? C2R_VMAX : 0) | (last_half ? C2R_KICK : 0),
                      c->vmax_bits, c->stream);
     return wait ? read_vmax(c) : QX_OK;
@@ -825,13 +856,19 @@ int qx_step_pass_count(const qx_ctx *c) { return valid_ctx(c) ? step_pass_count(
 // which strided tile-variant slot (op kind, stride class) a step pass uses
 static bool pass_slot(const qx_ctx *c, int p, int *kind, int *cls)
 {
+    // the class follows Engine::variant: inner stride above 4096 elements
+    const long long N = c->N;
     if (c->dim == 3) {
+        const bool t = tiled(c);
         switch (p) {
-        case 2: *kind = 2; *cls = c->N > 4096 ? 1 : 0; return true;   // drift along i0, inner N
-        case 6: *kind = 3; *cls = c->N > 4096 ? 1 : 0; return true;   // vt Poisson
+        case 1: *kind = 0; *cls = N > 4096 ? 1 : 0; return !t;   // fwd axis 1 (tile passes: fixed width)
+        case 2: *kind = 2; *cls = (t ? N * c->eng->tile_cols() : N * N) > 4096 ? 1 : 0; return true;  // drift
+        case 3: *kind = 1; *cls = N > 4096 ? 1 : 0; return !t;   // inv axis 1
+        case 6: *kind = 3; *cls = N > 4096 ? 1 : 0; return true;  // vt Poisson
         default: return false;
         }
     }
+    if (p == 1) { *kind = 2; *cls = N > 4096 ? 1 : 0; return true; }
     return false;
 }
 
@@ -874,9 +911,9 @@ qx_status qx_set_tile_variants(qx_ctx *c, const int *variants)
 
 const char *qx_step_pass_name(const qx_ctx *c, int p)
 {
-    static const char *n3[] = {"c2r_kick_fwd2_T", "fwd1", "drift0", "inv1", "inv2_T_density_r2c",
+    static const char *n3[] = {"c2r_kick_fwd2", "fwd1_tiles", "drift0", "inv1_tiles", "inv2_density_r2c",
                                "vt_fwd1", "vt_poisson0", "vt_inv1"};
-    static const char *n2[] = {"c2r_kick_fwd1_T", "drift0", "inv1_T_density_r2c", "vt_poisson0"};
+    static const char *n2[] = {"c2r_kick_fwd1", "drift0", "inv1_density_r2c", "vt_poisson0"};
     if (!valid_ctx(c) || p < 0 || p >= step_pass_count(c)) return "";
     return c->dim == 3 ? n3[p] : n2[p];
 }
@@ -892,16 +929,17 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     HIPCHK(hipMemcpyAsync(c->aux_vt, c->vt, (size_t)c->H * c->csz, hipMemcpyDeviceToDevice, c->stream));
     const MulArgs pm = poisson_args(c);
     MulArgs dm;
-    TRY(drift_args(c, ds, &dm, true));
-    TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+    TRY(drift_args(c, ds, &dm, tiled(c)));
+    if (tiled(c)) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+    if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     qx_status st = QX_OK;
     for (int p = 0; p < np && st == QX_OK; ++p) {
-        step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, dm, pm);  // warm
+        step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, dm, pm);  // warm
         if (hipEventRecord(e0, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
-        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, dm, pm);
+        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, dm, pm);
         if (hipEventRecord(e1, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
         float t = 0.f;
         if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
@@ -998,10 +1036,10 @@ qx_status leapfrog_enqueue(qx_ctx *c, double ds, int nsteps, int first_half, int
 qx_status update_potential_enqueue(qx_ctx *c)
 {
     CHECK_CTX(c);
-    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->rows, c->ph.pref, rho_bar_eff(c),
+    c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->vt, c->rows, c->N, c->ph.pref, rho_bar_eff(c),
                      c->stream);
-    vt_poisson(c, c->vt);
-    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->rows, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V,
+    TRY(vt_poisson(c, c->vt));
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->N, 0.0, nullptr, C2R_SRC_VT | C2R_STORE_V,
                      nullptr, c->stream);
     return QX_OK;
 }
@@ -1114,7 +1152,6 @@ struct EnsClass {
     int N = 0, B = 0;
     long long S = 0, rows = 0, H1 = 0, H = 0;
     void *psi = nullptr, *V = nullptr, *vt = nullptr;
-    void *psit = nullptr;                                      // k1-major copy of psi, [B][k1][i0]
     double2 *d_ptab = nullptr;                                 // per member phase tables [B][N]
     double *d_pfac = nullptr;                                  // per member
     double *d_kick_first = nullptr, *d_kick_mid = nullptr, *d_kick_last = nullptr;
@@ -1184,11 +1221,12 @@ MulArgs ens_drift_args(const EnsClass &k)
 {
     MulArgs m{};
     m.kind = MUL_DRIFT;
-    m.geo = GEO_PSIT2;
+    m.geo = GEO_PSI2;
     m.inv_s = 1.0 / (double)k.S;
     m.ptab = k.d_ptab;
     m.ptab_stride = k.N;
-    m.member_div = k.N;   // drift lines are member * N + k1
+    m.member_div = 1;
+    m.tcols = 1;
     return m;
 }
 MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
@@ -1206,10 +1244,10 @@ MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
 void ens_class_step(qx_ens *e, EnsClass &k, hipStream_t s)
 {
     const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
-    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid,
-                    C2R_SRC_VT | C2R_KICK | C2R_FWD | C2R_FWD_T, nullptr, s, k.psit);
-    k.eng->contig(k.psit, k.rows, 0, dm, s);
-    k.eng->r2c_rows(R2C_INV_DENSITY_T, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s, k.psit);
+    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_FWD,
+                    nullptr, s);
+    k.eng->strided(k.psi, k.B, k.N, 0, dm, s);
+    k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
     k.eng->contig(k.vt, k.B * k.H1, 0, pm, s);
 }
 
@@ -1319,12 +1357,12 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
             hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)
             return cleanup(fail(QX_EDEVICE, "stream creation failed"));
         void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_ptab, (void **)&k.d_pfac,
-                         (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last, &k.psit};
+                         (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
         const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
                              (size_t)(k.B * k.H) * e->csz, (size_t)k.N * e->csz,
                              sizeof(double2) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
-                             sizeof(double) * k.B, sizeof(double) * k.B, (size_t)(k.B * k.S) * e->csz};
-        for (int b = 0; b < 10; ++b)
+                             sizeof(double) * k.B, sizeof(double) * k.B};
+        for (int b = 0; b < 9; ++b)
             if (hipMalloc(bufs[b], sz[b]) != hipSuccess) {
                 *bufs[b] = nullptr;
                 return cleanup(fail(QX_ENOMEM, "ensemble allocation failed"));
@@ -1350,7 +1388,7 @@ qx_status qx_ens_destroy(qx_ens *e)
     for (auto &k : e->classes) {
         if (k.stream) (void)hipStreamSynchronize(k.stream);
         void *bufs[] = {k.psi, k.V, k.vt, k.eng ? k.eng->tw : nullptr, k.d_ptab, k.d_pfac,
-                        k.d_kick_first, k.d_kick_mid, k.d_kick_last, k.psit};
+                        k.d_kick_first, k.d_kick_mid, k.d_kick_last};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
         if (k.done) (void)hipEventDestroy(k.done);
@@ -1486,11 +1524,9 @@ qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_hal
         for (int s = 0; s < k.B; ++s) kf[s] = (first_half ? 0.5 : 1.0) * ds[k.members[s]];
         HIPCHK(hipMemcpy(k.d_kick_first, kf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
-        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_FWD | C2R_FWD_T,
-                        nullptr, e->stream, k.psit);
-        k.eng->contig(k.psit, k.rows, 0, dm, e->stream);
-        k.eng->r2c_rows(R2C_INV_DENSITY_T, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream,
-                        k.psit);
+        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_FWD, nullptr, e->stream);
+        k.eng->strided(k.psi, k.B, k.N, 0, dm, e->stream);
+        k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
         k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
     }
     TRY(ens_steady_steps(e, nsteps - 1));

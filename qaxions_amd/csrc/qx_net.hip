@@ -1,7 +1,14 @@
 // qx_net.hip -- learned coarse-step corrector: four periodic 3x3 convolutions
-// 2 -> 32 -> 32 -> 32 -> 2 (GELU after the first three) applied to
-// (Re psi, Im psi) of a 2D lattice, psi <- psi + net(psi).  No reference
-// counterpart.
+// 3 -> 32 -> 32 -> 32 -> 2 (GELU after the first three) applied to
+// (Re psi, Im psi, a V) of a 2D lattice, psi <- psi + s net(psi, a V).  No
+// reference counterpart.
+//
+// Why V is an input: the splitting error of a KDK step is a sum of nested
+// commutators of T = -laplacian/2 and V, with terms in grad V and the
+// Hessian of V.  V is a nonlocal functional of psi (a Poisson solve), so a
+// network with a 9 x 9 receptive field cannot form those terms from psi
+// alone; given V it can (they are local in V).  The lattice always holds the
+// V of the state being corrected.
 //
 // Numerics: weights are held in FP32 (master copy, Adam state) and rounded
 // to BF16 for every forward/backward use; the hidden activations are stored
@@ -10,8 +17,9 @@
 // (shifted input patches) x (tap weights), K = 9 taps x 32 channels; the
 // thin layers and the backward pass are FP32 vector kernels.
 //
-// Layouts: activations [P][C] (channel fastest, P = H*W row-major), so a
-// complex64 psi array is the [P][2] input/output; kernel weights
+// Layouts: activations [P][C] (channel fastest, P = H*W row-major); the
+// input is [P][3] (Re psi, Im psi, a V), the output a complex64 psi array
+// [P][2]; kernel weights
 // [tap][ci][co] with tap = 3*dy + dx reading x[(h+dy-1) mod H][(w+dx-1) mod W];
 // the flat parameter vector is, per layer, W[co][ci][dy][dx] then b[co].
 #include "../../include/qx.h"
@@ -50,8 +58,9 @@ qx_status nfail(qx_status s, const std::string &m)
     } while (0)
 
 constexpr int NL = 4;
-constexpr int CH[NL + 1] = {2, 32, 32, 32, 2};
-constexpr int NPARAM = 19682;
+constexpr int CH[NL + 1] = {3, 32, 32, 32, 2};
+constexpr int NPARAM = 19970;
+constexpr int CIN = 3;  // input channels
 
 __host__ __device__ constexpr int w_count(int l) { return CH[l] * CH[l + 1] * 9; }
 __host__ __device__ constexpr int layer_off(int l) { return l == 0 ? 0 : layer_off(l - 1) + w_count(l - 1) + CH[l]; }
@@ -93,7 +102,8 @@ __global__ void pack_kernel(const float *__restrict__ theta, bf16_t *__restrict_
 // vector convolution: one thread per pixel, all CO outputs
 //   IN = float (first layer, psi) or bf16 (hidden activations)
 //   MODE 0: forward, z = conv + b, a = gelu(z) -> bf16 (hidden layers)
-//   MODE 1: forward last layer: out = x + oscale (conv + b), float [P][2]
+//   MODE 1: forward last layer: out = x[:, 0:2] + oscale (conv + b), float
+//           [P][2]; x (the network input, x_res) has CIN channels per pixel
 //   MODE 2: input gradient of a hidden layer: d_in[p][c] = conv_T(dz) then
 //           multiplied by gelu'(z_prev[p][c]) (the producing layer's pre-act)
 //   MODE 3: input gradient without the GELU factor (gradient w.r.t. psi)
@@ -172,7 +182,7 @@ conv_vec(const IN *__restrict__ in, int H, int W, const bf16_t *__restrict__ wk,
     } else if constexpr (MODE == 1) {
         float *y = (float *)out;
 #pragma unroll
-        for (int o = 0; o < CO; ++o) y[p * CO + o] = fmaf(oscale, acc[o], x_res[p * CO + o]);
+        for (int o = 0; o < CO; ++o) y[p * CO + o] = fmaf(oscale, acc[o], x_res[p * CIN + o]);
     } else if constexpr (MODE == 2) {
         float *d = (float *)out;
 #pragma unroll
@@ -399,6 +409,18 @@ __global__ void adam_kernel(float *__restrict__ theta, float *__restrict__ m, fl
     }
 }
 
+// network input from a lattice: (Re psi, Im psi, vscale V)
+__global__ void input_kernel(const float *__restrict__ psi, const float *__restrict__ V, float vscale,
+                             long long P, float *__restrict__ x)
+{
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < P;
+         p += (long long)gridDim.x * blockDim.x) {
+        x[3 * p] = psi[2 * p];
+        x[3 * p + 1] = psi[2 * p + 1];
+        x[3 * p + 2] = vscale * V[p];
+    }
+}
+
 unsigned grid1(long long n, int t = 256)
 {
     long long b = (n + t - 1) / t;
@@ -420,9 +442,11 @@ struct qx_net {
     float *d0 = nullptr, *d1 = nullptr;  // gradient buffers [P][32]
     float *partial = nullptr;
     double *lpart = nullptr, *h_lpart = nullptr;
-    float *xbuf = nullptr, *tbuf = nullptr;  // host-array staging
+    float *xbuf = nullptr, *tbuf = nullptr;  // host-array staging ([P][3], [P][2])
+    float *xin = nullptr;    // input built from a lattice [P][3]
     int adam_t = 0;
     float out_scale = 1.0f;  // y = x + out_scale * net(x)
+    float in_scale = 1.0f;   // V channel = in_scale * V for lattice inputs
     bool packed = false;
     int wchunk = 512, wblocks = 0;
     void *save_psi = nullptr, *save_V = nullptr;  // fallback copies (lazy)
@@ -449,13 +473,14 @@ qx_status pack(qx_net *n)
 
 const float *bias(const qx_net *n, int l) { return n->theta + layer_off(l) + w_count(l); }
 
-// forward pass: x [P][2] float -> n->y = x + net(x); keep z for training
+// forward pass: x [P][3] float -> y [P][2] = x[:, 0:2] + s net(x); keep z
+// for training
 qx_status forward(qx_net *n, const float *x, float *y, bool keep, hipStream_t s = nullptr)
 {
     if (!s) s = n->stream;
     if (!n->packed) NTRY(pack(n));
     const unsigned gp = (unsigned)((n->P + 255) / 256);
-    conv_vec<2, 32, 0, float><<<gp, 256, 0, s>>>(x, n->H, n->W, n->wf + wofs(0), bias(n, 0),
+    conv_vec<CIN, 32, 0, float><<<gp, 256, 0, s>>>(x, n->H, n->W, n->wf + wofs(0), bias(n, 0),
                                                  keep ? n->z[0] : nullptr, n->a[0], nullptr, nullptr);
     const bool mfma = (n->W % 16) == 0;
     for (int l = 1; l <= 2; ++l) {
@@ -488,7 +513,7 @@ qx_status nsync(qx_net *n)
     return QX_OK;
 }
 
-// loss and full gradient into n->grad; x, t device [P][2]
+// loss and full gradient into n->grad; x device [P][3], t device [P][2]
 qx_status loss_grad(qx_net *n, const float *x, const float *t, double *loss)
 {
     NTRY(forward(n, x, n->y, true));
@@ -512,9 +537,9 @@ qx_status loss_grad(qx_net *n, const float *x, const float *t, double *loss)
         conv_vec<32, 32, 2, float><<<gp, 256, 0, s>>>(dz, n->H, n->W, n->wt + wofs(l), nullptr, nullptr, dn,
                                                       nullptr, n->z[l - 1]);
     }
-    // layer 0 (2 -> 32): dz0 in d1
-    wgrad_kernel<2, 32, float><<<wb, 256, 0, s>>>(x, n->d1, n->H, n->W, n->wchunk, n->partial);
-    wgrad_reduce<<<grid1(9 * 2 * 32 + 32), 256, 0, s>>>(n->partial, wb, 2, 32, layer_off(0), n->grad);
+    // layer 0 (3 -> 32): dz0 in d1
+    wgrad_kernel<CIN, 32, float><<<wb, 256, 0, s>>>(x, n->d1, n->H, n->W, n->wchunk, n->partial);
+    wgrad_reduce<<<grid1(9 * CIN * 32 + 32), 256, 0, s>>>(n->partial, wb, CIN, 32, layer_off(0), n->grad);
     if (loss) {
         NCHK(hipMemcpyAsync(n->h_lpart, n->lpart, sizeof(double) * gl, hipMemcpyDeviceToHost, s));
         NTRY(nsync(n));
@@ -561,8 +586,8 @@ qx_status qx_net_create(int H, int W, int device, qx_net **out)
         {(void **)&n->y, sizeof(float) * n->P * 2}, {(void **)&n->d0, sizeof(float) * n->P * 32},
         {(void **)&n->d1, sizeof(float) * n->P * 32},
         {(void **)&n->partial, sizeof(float) * (size_t)n->wblocks * (9 * 32 * 32 + 32)},
-        {(void **)&n->lpart, sizeof(double) * 4096}, {(void **)&n->xbuf, sizeof(float) * n->P * 2},
-        {(void **)&n->tbuf, sizeof(float) * n->P * 2}};
+        {(void **)&n->lpart, sizeof(double) * 4096}, {(void **)&n->xbuf, sizeof(float) * n->P * CIN},
+        {(void **)&n->tbuf, sizeof(float) * n->P * 2}, {(void **)&n->xin, sizeof(float) * n->P * CIN}};
     for (auto &x : al)
         if (hipMalloc(x.p, x.b) != hipSuccess) {
             *x.p = nullptr;
@@ -585,7 +610,8 @@ qx_status qx_net_destroy(qx_net *n)
     (void)hipSetDevice(n->device);
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     void *b[] = {n->theta, n->grad, n->m, n->v, n->wf, n->wt, n->z[0], n->z[1], n->z[2], n->a[0], n->a[1],
-                 n->a[2], n->y, n->d0, n->d1, n->partial, n->lpart, n->xbuf, n->tbuf, n->save_psi, n->save_V};
+                 n->a[2], n->y, n->d0, n->d1, n->partial, n->lpart, n->xbuf, n->tbuf, n->xin, n->save_psi,
+                 n->save_V};
     for (void *p : b)
         if (p) (void)hipFree(p);
     if (n->h_lpart) (void)hipHostFree(n->h_lpart);
@@ -626,7 +652,7 @@ qx_status qx_net_forward_host(qx_net *n, const float *x, float *y)
 {
     if (!n || !x || !y) return nfail(QX_EINVAL, "null argument");
     NCHK(hipSetDevice(n->device));
-    NCHK(hipMemcpy(n->xbuf, x, sizeof(float) * n->P * 2, hipMemcpyHostToDevice));
+    NCHK(hipMemcpy(n->xbuf, x, sizeof(float) * n->P * CIN, hipMemcpyHostToDevice));
     NTRY(forward(n, n->xbuf, n->y, false));
     NTRY(nsync(n));
     NCHK(hipMemcpy(y, n->y, sizeof(float) * n->P * 2, hipMemcpyDeviceToHost));
@@ -647,7 +673,7 @@ qx_status qx_net_loss_grad_host(qx_net *n, const float *x, const float *t, doubl
 {
     if (!n || !x || !t) return nfail(QX_EINVAL, "null argument");
     NCHK(hipSetDevice(n->device));
-    NCHK(hipMemcpy(n->xbuf, x, sizeof(float) * n->P * 2, hipMemcpyHostToDevice));
+    NCHK(hipMemcpy(n->xbuf, x, sizeof(float) * n->P * CIN, hipMemcpyHostToDevice));
     NCHK(hipMemcpy(n->tbuf, t, sizeof(float) * n->P * 2, hipMemcpyHostToDevice));
     return qx_net_loss_grad(n, n->xbuf, n->tbuf, loss, grad);
 }
@@ -675,6 +701,24 @@ qx_status qx_net_train_step(qx_net *n, const void *x_dev, const void *t_dev, flo
 
 void *qx_net_stream(qx_net *n) { return n ? (void *)n->stream : nullptr; }
 
+static qx_status check_ctx(qx_net *n, qx_ctx *ctx, float **psi, float **V);
+static void lattice_input(qx_net *n, const float *psi, const float *V, hipStream_t s);
+
+// one training step on a pair of lattices: input (psi, V) of `coarse`,
+// target psi of `fine`
+qx_status qx_net_train_step_lattices(qx_net *n, qx_ctx *coarse, qx_ctx *fine, float lr, double *loss)
+{
+    float *pc, *vc, *pf, *vf;
+    NTRY(check_ctx(n, coarse, &pc, &vc));
+    NTRY(check_ctx(n, fine, &pf, &vf));
+    NCHK(hipSetDevice(n->device));
+    NCHK(hipStreamSynchronize((hipStream_t)qx_stream(coarse)));
+    NCHK(hipStreamSynchronize((hipStream_t)qx_stream(fine)));
+    lattice_input(n, pc, vc, n->stream);
+    NTRY(loss_grad(n, n->xin, pf, loss));
+    return qx_net_adam_step(n, lr, 0.9f, 0.999f, 1e-8f);
+}
+
 // mean((x - t)^2) over a pair: the loss of an identity corrector
 qx_status qx_net_pair_mse(qx_net *n, const void *x_dev, const void *t_dev, double *mse)
 {
@@ -699,6 +743,14 @@ qx_status qx_net_set_output_scale(qx_net *n, float s)
 }
 float qx_net_output_scale(const qx_net *n) { return n ? n->out_scale : 0.0f; }
 
+qx_status qx_net_set_input_scale(qx_net *n, float s)
+{
+    if (!n || !(s > 0.0f)) return nfail(QX_EINVAL, "input scale must be positive");
+    n->in_scale = s;
+    return QX_OK;
+}
+float qx_net_input_scale(const qx_net *n) { return n ? n->in_scale : 0.0f; }
+
 static qx_status check_ctx(qx_net *n, qx_ctx *ctx, float **psi, float **V)
 {
     void *p = nullptr, *v = nullptr;
@@ -711,13 +763,20 @@ static qx_status check_ctx(qx_net *n, qx_ctx *ctx, float **psi, float **V)
     return QX_OK;
 }
 
-// psi <- psi + net(psi) on a 2D FP32 lattice; V is not updated
+static void lattice_input(qx_net *n, const float *psi, const float *V, hipStream_t s)
+{
+    input_kernel<<<grid1(n->P), 256, 0, s>>>(psi, V, n->in_scale, n->P, n->xin);
+}
+
+// psi <- psi + net(psi, V) on a 2D FP32 lattice; V is not updated
 qx_status qx_net_correct(qx_net *n, qx_ctx *ctx)
 {
     float *psi, *V;
     NTRY(check_ctx(n, ctx, &psi, &V));
     NCHK(hipSetDevice(n->device));
-    NTRY(forward(n, psi, n->y, false));
+    NCHK(hipStreamSynchronize((hipStream_t)qx_stream(ctx)));
+    lattice_input(n, psi, V, n->stream);
+    NTRY(forward(n, n->xin, n->y, false));
     NCHK(hipMemcpyAsync(psi, n->y, sizeof(float) * n->P * 2, hipMemcpyDeviceToDevice, n->stream));
     return nsync(n);
 }
@@ -748,7 +807,8 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
         NCHK(hipMemcpyAsync(n->save_V, V, bv, hipMemcpyDeviceToDevice, s));
         for (int i = 0; i < chunk; ++i) {
             if (qx::leapfrog_enqueue(ctx, k * dt, 1, 1, 1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
-            NTRY(forward(n, psi, n->y, false, s));
+            lattice_input(n, psi, V, s);
+            NTRY(forward(n, n->xin, n->y, false, s));
             NCHK(hipMemcpyAsync(psi, n->y, bp, hipMemcpyDeviceToDevice, s));
             if (qx::update_potential_enqueue(ctx) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
         }

@@ -1,10 +1,10 @@
 """Learned coarse stepping: training pairs from the solver, GPU training of the
 corrector and its evaluation against plain fine and coarse stepping.
 
-A pair is (psi after one coarse KDK step of k*dt, psi after k fine KDK steps
-of dt) from the same state; states are taken along fine trajectories that
-start from Gaussian-random-field ICs.  Everything runs on the device; only
-scalars come back to the host.
+A pair is ((psi, V) after one coarse KDK step of k*dt, psi after k fine KDK
+steps of dt) from the same state; states are taken along fine trajectories
+that start from Gaussian-random-field ICs.  Everything runs on the device;
+only scalars come back to the host.
 """
 from __future__ import annotations
 
@@ -19,7 +19,7 @@ def init_params(seed=0, last_scale=1e-3):
     """He-style initial parameters; the last layer starts near zero, so the
     corrector starts close to the identity on psi."""
     rng = np.random.default_rng(seed)
-    chans = (2, 32, 32, 32, 2)
+    chans = (3, 32, 32, 32, 2)
     out = []
     for i, (ci, co) in enumerate(zip(chans[:-1], chans[1:])):
         w = rng.standard_normal((co, ci, 3, 3)) * np.sqrt(2.0 / (ci * 9))
@@ -52,9 +52,12 @@ def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, 
             fine.copy_from(traj)
             fine.leapfrog(dt, k)
             if not scale_set:
-                # the network predicts the correction in units of its typical size
+                # the network predicts the correction in units of its typical
+                # size and sees V in units of its spread
                 d = fine.psi - coarse.psi
                 net.output_scale = float(np.sqrt(np.mean(np.abs(d) ** 2) / 2.0))
+                v = coarse.V
+                net.input_scale = float(1.0 / max(np.std(v), 1e-30))
                 scale_set = True
             lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(total - 1, 1))
             base = net.pair_mse(coarse, fine)

@@ -7,7 +7,7 @@ from oracle import corrector as cn
 
 
 def test_param_count():
-    assert cn.n_params() == 2 * 32 * 9 + 32 + 2 * (32 * 32 * 9 + 32) + 32 * 2 * 9 + 2 == 19682
+    assert cn.n_params() == 3 * 32 * 9 + 32 + 2 * (32 * 32 * 9 + 32) + 32 * 2 * 9 + 2 == 19970
 
 
 def test_conv_matches_direct_sum():
@@ -29,11 +29,13 @@ def test_gradient_finite_differences():
     rng = np.random.default_rng(1)
     H = W = 6
     theta = cn.init_params(3, last_scale=0.3)
-    x = rng.standard_normal((H, W, 2))
+    x = rng.standard_normal((H, W, 3))
     t = rng.standard_normal((H, W, 2))
     loss, g = cn.loss_and_grad(theta, x, t)
     idx = rng.choice(theta.size, 40, replace=False)
-    idx = np.concatenate([idx, [0, 607, 608, 19681, 19680, 19102]])
+    # first and last weight and first bias of layer 0, the V channel's first
+    # tap, the last layer's weights and biases
+    idx = np.concatenate([idx, [0, 863, 864, 18, 19969, 19968, 19390]])
     for i in idx:
         e = np.zeros_like(theta)
         h = 1e-5 * max(1.0, abs(theta[i]))

@@ -19,13 +19,14 @@ def test_forward_matches_oracle(qx, H, W):
     print(f"[case] corrector forward {H}x{W}", flush=True)
     rng = np.random.default_rng(H * 100 + W)
     theta = cn.init_params(5, last_scale=0.5)
-    x = rng.standard_normal((H, W, 2))
+    x = rng.standard_normal((H, W, 3))
     net = qx.Corrector(H, W, params=theta.astype(np.float32))
     y = net.forward(x.astype(np.float32)).astype(np.float64)
     theta32 = theta.astype(np.float32).astype(np.float64)
-    want = x.astype(np.float32).astype(np.float64) + cn.forward(theta32, x.astype(np.float32).astype(np.float64), bf16=True)
-    # compare the network part (y - x), where rounding points decide agreement
-    assert rel(y - x, want - x) < 1e-2
+    x32 = x.astype(np.float32).astype(np.float64)
+    want = x32[..., :2] + cn.forward(theta32, x32, bf16=True)
+    # compare the network part (y - psi), where rounding points decide agreement
+    assert rel(y - x32[..., :2], want - x32[..., :2]) < 1e-2
 
 
 @pytest.mark.parametrize("H,W", [(16, 16), (12, 20)])
@@ -33,8 +34,8 @@ def test_gradient_matches_oracle(qx, H, W):
     print(f"[case] corrector gradient {H}x{W}", flush=True)
     rng = np.random.default_rng(7)
     theta = cn.init_params(9, last_scale=0.5).astype(np.float32)
-    x = rng.standard_normal((H, W, 2)).astype(np.float32)
-    t = (x + 0.1 * rng.standard_normal((H, W, 2))).astype(np.float32)
+    x = rng.standard_normal((H, W, 3)).astype(np.float32)
+    t = (x[..., :2] + 0.1 * rng.standard_normal((H, W, 2))).astype(np.float32)
     net = qx.Corrector(H, W, params=theta)
     net.output_scale = 0.25
     loss, g = net.loss_grad(x, t)
@@ -54,8 +55,8 @@ def test_gradient_matches_oracle(qx, H, W):
 def test_adam_step_lowers_loss(qx):
     rng = np.random.default_rng(3)
     H = W = 16
-    x = rng.standard_normal((H, W, 2)).astype(np.float32)
-    t = (x + 0.05 * np.roll(x, 1, axis=0)).astype(np.float32)
+    x = rng.standard_normal((H, W, 3)).astype(np.float32)
+    t = (x[..., :2] + 0.05 * np.roll(x[..., :2], 1, axis=0) + 0.05 * x[..., 2:]).astype(np.float32)
     net = qx.Corrector(H, W, params=cn.init_params(1).astype(np.float32))
     l0, _ = net.loss_grad(x, t)
     for _ in range(30):
@@ -85,7 +86,9 @@ def test_forced_fallback_equals_fine_steps(qx):
 
 def test_trained_corrector_beats_plain_coarse(qx):
     """Trained on solver pairs (seeds disjoint from evaluation), corrected
-    coarse stepping is closer to the fine solution than coarse stepping alone."""
+    coarse stepping is closer to the fine solution than coarse stepping
+    alone.  The input includes V: the splitting error has grad V and Hessian V
+    terms, which psi alone does not determine locally."""
     from qaxions_amd import learned
     # self-gravity strong enough (pref = 100) that the coarse step's splitting
     # error stands well above FP32 round-off; at pref = 1 the two agree to
