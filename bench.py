@@ -57,6 +57,18 @@ class HipEvents:
         return t.value
 
 
+def measured_traffic(pass_name, path=os.path.join(ROOT, "profiles", "kdk512_traffic.json")):
+    """HBM bytes per launch of a pass of the 512^3 FP32 step, from the
+    committed FETCH_SIZE / WRITE_SIZE counter runs (scaled by the copy
+    calibration recorded in the same file); None when not measured."""
+    try:
+        with open(path) as f:
+            rec = json.load(f)["passes"][pass_name]
+        return rec["fetch_bytes"] + rec["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(lat, N, L, dt, steps, threads):
     """FP64 oracle (oracle/qx_oracle.c, OpenMP) on the host: `steps` merged-kick
     steps of the same lattice and IC, timed with a wall clock."""
@@ -80,7 +92,7 @@ def main():
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--dim", type=int, default=3)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pass-reps", type=int, default=10)
@@ -167,7 +179,7 @@ def main():
                      "achieved": pbytes[dom] / (pms[dom] * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": pbytes[dom] / (pms[dom] * 1e-3) / HBM_PEAK,
-                     "traffic": None,
+                     "traffic": measured_traffic(names[dom]) if (N, dim, args.precision) == (512, 3, "fp32") else None,
                      "bytes_per_launch": pbytes[dom], "ms_per_launch": pms[dom]},
         "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_site": B_site,
                           "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
