@@ -276,7 +276,7 @@ def bench_learned(qx, args):
                                 log_every=1000, pref=pref)
     t_train = time.perf_counter() - t0
     res = learned.evaluate(net, N, L, dt, k, seed=0, n_coarse=args.eval_steps, check_every=10,
-                           mass_tol=1e-6, timing_reps=2, pref=pref)
+                           mass_tol=1e-6, timing_reps=2, pref=pref, warm_steps=400)
     out = {"metric": "learned coarse stepping at 512^2: speed-up over fine stepping and errors",
            "value": res["speedup_learned_vs_fine"], "unit": "x", "n_gpus": 1,
            "train_pairs": int(len(losses)), "train_seconds": t_train,

@@ -78,14 +78,18 @@ def _rel(a, b):
 
 
 def evaluate(net, N, L, dt, k, seed, n_coarse, check_every=10, mass_tol=1e-6, ref_refine=4,
-             pref=1.0, timing_reps=1):
+             pref=1.0, timing_reps=1, warm_steps=0):
     """Errors after n_coarse coarse intervals (time n_coarse*k*dt) against a
     reference run with dt/ref_refine, for plain fine stepping (dt), plain
     coarse stepping (k dt) and corrected coarse stepping; plus device wall
-    times of the fine and corrected runs."""
+    times of the fine and corrected runs.  The runs start after warm_steps
+    fine steps from the IC (a GRF IC is real at t = 0, unlike the states the
+    pairs are drawn from)."""
     lat0 = Lattice(2, N, L, "fp32", pref=pref)
     lat0.ic_grf(int(seed))
     lat0.update_potential()
+    if warm_steps:
+        lat0.leapfrog(dt, int(warm_steps))
     runs = {}
 
     def run(name, fn):
