@@ -140,7 +140,7 @@ qx_status qx_ic_blobs(qx_ctx *ctx, const double *table, int nblobs, double bkg);
 
 /* ---- measurement (no reference counterpart; replaces the reference's
  * PROFILE/FLOP_COUNT section timers, src/profiler.h:158-185) -------------- */
-/* passes of one fused steady-state KDK step (8 in 3D, 4 in 2D) */
+/* passes of one fused steady-state KDK step (7 in 3D, 4 in 2D) */
 int qx_step_pass_count(const qx_ctx *ctx);
 const char *qx_step_pass_name(const qx_ctx *ctx, int pass);
 /* Times each pass alone, `reps` launches back to back on scratch copies of

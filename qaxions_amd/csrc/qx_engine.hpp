@@ -24,13 +24,21 @@ struct Engine {
     // multiple of it
     virtual void strided_tiles(void *data, void *tiles, long long outer, long long inner, int op,
                                hipStream_t s) = 0;
+    // drift along the strided axis (fwd, the axis' factor, inv), data columns
+    // -> tiles
+    virtual void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner,
+                                     const MulArgs &m, hipStream_t s) = 0;
     virtual int tile_cols() const = 0;
     // rows of `rpm` rows per member (rpm = rows for a single lattice)
     virtual void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows,
                           long long rpm, double pref, double rho_bar, hipStream_t s) = 0;
+    // C2R_SRC_TILES: psi read from `tiles`; C2R_DRIFT: the last axis' drift
+    // with the per-axis table ptab (members ptab_stride apart) and 1/N
     virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm,
                           double kick_dt, const double *kick_member, int flags,
-                          unsigned long long *vmax, hipStream_t s) = 0;
+                          unsigned long long *vmax, hipStream_t s, const void *tiles = nullptr,
+                          const double2 *ptab = nullptr, const float2 *ftab = nullptr,
+                          long long ptab_stride = 0, double inv_n = 0.0) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
     // tile variant per strided pass kind (0 = full tile, 1 = half tile),
     // indexed by [op kind][stride class]; set by qx_tune
@@ -87,6 +95,16 @@ template <int N, typename R> struct EngineT final : Engine {
         else
             strided_fft<N, R, OP_INV, TC, XM_LOAD_TILES><<<blocks, threads, 0, s>>>(d, inner, ntiles, none, t, l, outer);
     }
+    void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
+                             hipStream_t s) override
+    {
+        // 16-column tiles take two columns per thread (512 threads)
+        constexpr int TC = G::TC, CPT = (TC >= 16 && TC % 2 == 0) ? 2 : 1;
+        const long long ntiles = inner / TC;
+        const unsigned blocks = (unsigned)(outer * ntiles);
+        strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
+            (C *)data, inner, ntiles, m, (const C *)tw, (C *)tiles, outer);
+    }
     int tile_cols() const override { return G::TC; }
 
     template <int TC>
@@ -99,6 +117,8 @@ template <int N, typename R> struct EngineT final : Engine {
         const C *t = (const C *)tw;
         if (op < 0) strided_fft<N, R, OP_FWD, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
         else if (op > 0) strided_fft<N, R, OP_INV, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        else if (m.kind == MUL_DRIFT && m.geo == GEO_AXIS)
+            strided_fft<N, R, OP_DRIFT_AXIS, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
         else if (m.kind == MUL_DRIFT) strided_fft<N, R, OP_DRIFT, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
         else strided_fft<N, R, OP_POISSON, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
     }
@@ -120,16 +140,21 @@ template <int N, typename R> struct EngineT final : Engine {
             qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else if (mode == R2C_DENSITY)
             qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
-        else
+        else if (mode == R2C_INV_DENSITY)
             qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+        else
+            qx::r2c_rows<N, R, R2C_DENSITY_TILES><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
-                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s) override
+                  const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s,
+                  const void *tiles, const double2 *ptab, const float2 *ftab, long long ptab_stride,
+                  double inv_n) override
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
                                                              (R)kick_dt, kick_member, flags, vmax,
-                                                             (const C *)tw);
+                                                             (const C *)tw, (const C *)tiles, ptab, ftab,
+                                                             ptab_stride, inv_n);
     }
 };
 

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace qx {
 
 template <typename R> struct cplx;
@@ -123,6 +125,20 @@ struct LdsCols {    // [pos][column], columns interleaved
     __device__ __forceinline__ static int at(int pos, int tc) { return pos * tc; }
 };
 
+// Synchronisation of a line's LDS exchange.  A contiguous line is held by
+// TPL consecutive threads; with TPL <= 64 those are lanes of one wavefront,
+// whose LDS operations complete in issue order, so waiting for this wave's
+// own LDS operations (and keeping the compiler from moving memory accesses
+// across the wait) is enough.  Column-interleaved lines span wavefronts and
+// take a block barrier.
+template <int N, typename L> __device__ __forceinline__ void line_sync()
+{
+    if constexpr (std::is_same<L, LdsContig>::value && Plan<N>::TPL <= 64)
+        __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+    else
+        __syncthreads();
+}
+
 // One Stockham stage.  v holds x[t + q TPL]; on exit (not last) v holds the
 // next stage's inputs, read back from LDS; (last) v holds y[t + q TPL].
 template <int N, int S, int DIR, typename L, typename C>
@@ -163,10 +179,10 @@ __device__ __forceinline__ void fft_stage(C (&v)[Plan<N>::E], int t, C *line, in
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = out[q];
     } else {
-        __syncthreads();
+        line_sync<N, L>();
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = line[L::at(t + q * TPL, tc)];
-        __syncthreads();
+        line_sync<N, L>();
     }
 }
 
@@ -180,7 +196,8 @@ __device__ __forceinline__ void stage_loop(C (&v)[Plan<N>::E], int t, C *line, i
 }
 
 // Full N-point transform of the line held in v.  All threads of the block
-// must call it (it contains block barriers).  `line` is this line's LDS
+// must call it (column lines and lines of more than 64 threads synchronise
+// with block barriers).  `line` is this line's LDS
 // area, `tc` the column interleave for LdsCols, `tw` the LDS twiddle table
 // tw[m] = e^{-2 pi i m/N}.
 template <int N, int DIR, typename L, typename C>

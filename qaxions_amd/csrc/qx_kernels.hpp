@@ -61,7 +61,9 @@ enum { MUL_NONE = 0, MUL_DRIFT = 1, MUL_POISSON = 2 };
 // pass operations (template argument): plain transforms, or forward,
 // pointwise factor and inverse in one pass; separate instantiations keep the
 // factor code out of the plain passes and name each pass in profiles
-enum { OP_FWD = -1, OP_INV = 1, OP_DRIFT = 0, OP_POISSON = 2 };
+enum { OP_FWD = -1, OP_INV = 1, OP_DRIFT = 0, OP_POISSON = 2, OP_DRIFT_AXIS = 3 };
+// OP_DRIFT_AXIS: one axis' drift (GEO_AXIS), the factor of the line position
+// only; FP32 lattices read it from the precomputed FP32 table ftab
 // geometry of the multiplied pass: how the 3 (or 2) integer frequencies of
 // an element follow from (line position p, column/line index c, outer o)
 enum {
@@ -71,6 +73,7 @@ enum {
     GEO_VT2 = 3,   // vt 2D, contiguous lines along i0, line index = k1
     GEO_PSIT3 = 4, // 3D psi in column tiles [k2 / TT][i0][k1][k2 % TT] (TT = tcols):
                    // axis i0 strided, o = k2 / TT, c = k1 * TT + k2 % TT
+    GEO_AXIS = 5,  // one axis' own factor: the table entry of the line position
 };
 
 struct MulArgs {
@@ -80,6 +83,9 @@ struct MulArgs {
     // (N entries per member, members ptab_stride apart); the factor of a mode is
     // the product over its axes, formed in FP64 and rounded once
     const double2 *ptab;
+    // FP32 lattices, one axis' drift (GEO_AXIS): e^{i theta} - 1 of each
+    // table entry, formed in FP64 and rounded once (same members layout)
+    const float2 *ftab;
     long long ptab_stride;
     double inv_s;    // 1/S
     double pfac;     // Poisson: 1/((2 pi / L)^2 S)
@@ -95,9 +101,21 @@ struct MulArgs {
 __device__ __forceinline__ MulArgs member_args(const MulArgs &a, long long member)
 {
     MulArgs m = a;
-    if (a.kind == MUL_DRIFT) m.ptab = a.ptab + member * a.ptab_stride;
+    if (a.kind == MUL_DRIFT) {
+        m.ptab = a.ptab + member * a.ptab_stride;
+        if (a.ftab) m.ftab = a.ftab + member * a.ptab_stride;
+    }
     else if (a.member_par) m.pfac = a.member_par[member];
     return m;
+}
+
+// index of (row = i0*N + i1, i2) of a 3D lattice in its column tiles
+// [i2/TT][i0][i1][i2%TT] (TT columns per tile)
+template <int N, int TT>
+__device__ __forceinline__ long long tile_index(long long row, int i2)
+{
+    const long long i0 = row / N, i1 = row - i0 * N;
+    return ((long long)(i2 / TT) * N + i0) * (long long)(N * TT) + i1 * TT + (i2 % TT);
 }
 
 // index of (row, k_last) in vt for members of `rpm` rows stored one after
@@ -173,6 +191,13 @@ __device__ __forceinline__ float2 drift_factor(const double2 &tp, const double2 
     const float fr = (float)(f.x - 1.0), fi = (float)f.y;
     const float tr = fmaf(x.x, fr, -x.y * fi), ti = fmaf(x.x, fi, x.y * fr);
     const float s = (float)inv_s;
+    return float2{(x.x + tr) * s, (x.y + ti) * s};
+}
+// one axis' drift factor in FP32 from its precomputed e^{i theta} - 1 (fm1):
+// the same arithmetic as drift_factor with a unit column factor
+__device__ __forceinline__ float2 drift_axis(float2 fm1, float s, float2 x)
+{
+    const float tr = fmaf(x.x, fm1.x, -x.y * fm1.y), ti = fmaf(x.x, fm1.y, x.y * fm1.x);
     return float2{(x.x + tr) * s, (x.y + ti) * s};
 }
 __device__ __forceinline__ double2 drift_factor(const double2 &tp, const double2 &cf, double inv_s, double2 x)
@@ -268,76 +293,102 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
 // ---------------------------------------------------------------------------
 enum { XM_NONE = 0, XM_STORE_TILES = 1, XM_LOAD_TILES = 2 };
 
-template <int N, typename R, int OP, int TC, int XM = XM_NONE>  // OP_FWD, OP_INV, OP_DRIFT or OP_POISSON
-__global__ void __launch_bounds__((TC * Plan<N>::TPL))
+// CPT columns per thread: a block of TC/CPT thread columns transforms its
+// CPT columns one after another through one LDS area (CPT = 2 keeps a
+// 16-column tile's 128-byte row segments with 512 threads per block).
+template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1>  // OP_FWD, OP_INV, OP_DRIFT(_AXIS) or OP_POISSON
+__global__ void __launch_bounds__(((TC / CPT) * Plan<N>::TPL))
 strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
             MulArgs mul, const typename cplx<R>::T *__restrict__ twg,
             typename cplx<R>::T *__restrict__ tiles, long long outer)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
-    __shared__ C lds[N * TC];
+    constexpr int TW = TC / CPT;
+    __shared__ C lds[N * TW];
     __shared__ C tw[N];
-    // the drift's phase table is staged in LDS when small (N <= 1024)
-    constexpr bool PT_LDS = (OP == OP_DRIFT) && N <= 1024;
-    __shared__ double2 ptab_lds[PT_LDS ? N : 1];
+    // the drift's phase table is staged in LDS when small (N <= 1024); an
+    // FP32 single-axis drift stages its FP32 table
+    constexpr bool FAX = OP == OP_DRIFT_AXIS && std::is_same<R, float>::value;
+    constexpr bool PT_LDS = (OP == OP_DRIFT || OP == OP_DRIFT_AXIS) && N <= 1024;
+    __shared__ double2 ptab_lds[PT_LDS ? (FAX ? N / 2 : N) : 1];
     load_tw<N>(tw, twg);
-    const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+    const int c = threadIdx.x % TW, t = threadIdx.x / TW;
     const long long b = xcd_block();
     const long long o = b / ntiles;
-    const long long col = (b % ntiles) * TC + c;
-    const bool ok = col < inner;
+    const long long col0 = (b % ntiles) * TC;
     const MulArgs ma = member_args(mul, o / (mul.member_div > 0 ? mul.member_div : 1));
     const double2 *ptab = ma.ptab;
-    if constexpr (PT_LDS) {
+    const float2 *ftab = ma.ftab;
+    if constexpr (PT_LDS && FAX) {
+        float2 *fl = (float2 *)ptab_lds;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = ma.ftab[i];
+        ftab = fl;
+    } else if constexpr (PT_LDS) {
         for (int i = threadIdx.x; i < N; i += blockDim.x) ptab_lds[i] = ma.ptab[i];
         ptab = ptab_lds;
     }
-    C *base = data + o * (long long)N * inner + col;
-    C *tile = tiles + ((b % ntiles) * outer + o) * (long long)N * TC + c;
-    C v[P::E];
-    if constexpr (XM == XM_LOAD_TILES) {
+    C *const tile0 = tiles + ((b % ntiles) * outer + o) * (long long)N * TC + c;
+    C *const base0 = data + o * (long long)N * inner + col0 + c;
+    C v[CPT][P::E];
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = ok ? tile[(t + q * P::TPL) * TC] : C{0, 0};
-    } else {
+    for (int j = 0; j < CPT; ++j) {
+        const bool ok = col0 + c + j * TW < inner;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = ok ? base[(long long)(t + q * P::TPL) * inner] : C{0, 0};
+        for (int q = 0; q < P::E; ++q) {
+            if constexpr (XM == XM_LOAD_TILES) v[j][q] = ok ? tile0[(t + q * P::TPL) * TC + j * TW] : C{0, 0};
+            else v[j][q] = ok ? base0[(long long)(t + q * P::TPL) * inner + j * TW] : C{0, 0};
+        }
     }
     __syncthreads();
-    if constexpr (OP == OP_FWD) {
-        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-    } else if constexpr (OP == OP_INV) {
-        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
-    } else {
-        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-        const int cc = ok ? (int)col : 0;
-        if constexpr (OP == OP_DRIFT) {
-            double2 cf;
-            if (mul.geo == GEO_PSI3) cf = zmul(ptab[cc / N], ptab[cc % N]);
-            else if (mul.geo == GEO_PSIT3)
-                cf = zmul(ptab[cc / mul.tcols], ptab[(int)o * mul.tcols + cc % mul.tcols]);
-            else cf = ptab[cc];
 #pragma unroll
-            for (int q = 0; q < P::E; ++q) v[q] = drift_factor(ptab[t + q * P::TPL], cf, ma.inv_s, v[q]);
-        } else {  // GEO_VT3 Poisson
-            const int a = foldi(cc, N);
-            const int mc = a * a + (int)(o * o);
+    for (int j = 0; j < CPT; ++j) {
+        const long long col = col0 + c + j * TW;
+        const bool ok = col < inner;
+        if constexpr (OP == OP_FWD) {
+            fft_line<N, -1, LdsCols>(v[j], t, lds + c, TW, tw);
+        } else if constexpr (OP == OP_INV) {
+            fft_line<N, +1, LdsCols>(v[j], t, lds + c, TW, tw);
+        } else {
+            fft_line<N, -1, LdsCols>(v[j], t, lds + c, TW, tw);
+            const int cc = ok ? (int)col : 0;
+            if constexpr (FAX) {
+                const float s = (float)ma.inv_s;
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) v[j][q] = drift_axis(ftab[t + q * P::TPL], s, v[j][q]);
+            } else if constexpr (OP == OP_DRIFT_AXIS) {
+                const double2 one{1.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) v[j][q] = drift_factor(ptab[t + q * P::TPL], one, ma.inv_s, v[j][q]);
+            } else if constexpr (OP == OP_DRIFT) {
+                double2 cf;
+                if (mul.geo == GEO_PSI3) cf = zmul(ptab[cc / N], ptab[cc % N]);
+                else if (mul.geo == GEO_PSIT3)
+                    cf = zmul(ptab[cc / mul.tcols], ptab[(int)o * mul.tcols + cc % mul.tcols]);
+                else cf = ptab[cc];
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) v[j][q] = drift_factor(ptab[t + q * P::TPL], cf, ma.inv_s, v[j][q]);
+            } else {  // GEO_VT3 Poisson
+                const int a = foldi(cc, N);
+                const int mc = a * a + (int)(o * o);
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    const int n0 = foldi(t + q * P::TPL, N);
+                    v[j][q] = poisson_factor<R>(ma, n0 * n0 + mc, v[j][q]);
+                }
+            }
+            fft_line<N, +1, LdsCols>(v[j], t, lds + c, TW, tw);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        if (col0 + c + j * TW < inner) {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
-                const int n0 = foldi(t + q * P::TPL, N);
-                v[q] = poisson_factor<R>(ma, n0 * n0 + mc, v[q]);
+                if constexpr (XM == XM_STORE_TILES) tile0[(t + q * P::TPL) * TC + j * TW] = v[j][q];
+                else base0[(long long)(t + q * P::TPL) * inner + j * TW] = v[j][q];
             }
         }
-        fft_line<N, +1, LdsCols>(v, t, lds + c, TC, tw);
-    }
-    if constexpr (XM == XM_STORE_TILES) {
-        if (ok) {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) tile[(t + q * P::TPL) * TC] = v[q];
-        }
-    } else if (ok) {
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) base[(long long)(t + q * P::TPL) * inner] = v[q];
     }
 }
 
@@ -349,6 +400,8 @@ enum {
     R2C_DENSITY = 1,   // input: psi rows, rho = pref (|psi|^2 - rho_bar)
     R2C_INV_DENSITY = 2,  // input: psi rows transformed along the last axis;
                           // inverse-transform, store psi, then as DENSITY
+    R2C_DENSITY_TILES = 3,  // as DENSITY, psi read from a 3D lattice's column
+                            // tiles (the `psi` argument; see tile_index)
 };
 
 template <int N, typename R, int MODE>
@@ -383,8 +436,13 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int pos = t + q * P::TPL;
-            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+            if constexpr (MODE == R2C_DENSITY_TILES) {
+                a[q] = oka ? psi[tile_index<N, G::TC>(ra, pos)] : C{0, 0};
+                b[q] = okb ? psi[tile_index<N, G::TC>(rb, pos)] : C{0, 0};
+            } else {
+                a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+                b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+            }
         }
         __syncthreads();
         if (MODE == R2C_INV_DENSITY) {
@@ -448,7 +506,10 @@ enum {
     C2R_STORE_V = 2,    // write the real rows to V
     C2R_VMAX = 4,       // fold max|V| into *vmax_bits
     C2R_KICK = 8,       // psi <- psi exp(-i kick_dt V)
-    C2R_FWD = 16,       // forward-transform psi along the last axis
+    C2R_DRIFT = 16,     // drift along the last axis: forward transform, the
+                        // axis' factor ptab[k] / N, inverse transform
+    C2R_SRC_TILES = 32, // psi read from a 3D lattice's column tiles (`tiles`,
+                        // see tile_index); psi rows are written
 };
 
 template <int N, typename R>
@@ -456,7 +517,9 @@ __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
          const double *__restrict__ kick_member, int flags,
-         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg)
+         unsigned long long *__restrict__ vmax_bits, const typename cplx<R>::T *__restrict__ twg,
+         const typename cplx<R>::T *__restrict__ tiles, const double2 *__restrict__ ptab,
+         const float2 *__restrict__ ftab, long long ptab_stride, double inv_n)
 {
     using C = typename cplx<R>::T;
     using P = Plan<N>;
@@ -472,14 +535,23 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
     // psi's rows first: their loads overlap the potential's transform
-    const bool upd = (flags & (C2R_KICK | C2R_FWD)) != 0;
+    const bool upd = (flags & (C2R_KICK | C2R_DRIFT | C2R_SRC_TILES)) != 0;
     C a[P::E], b[P::E];
     if (upd) {
+        if (flags & C2R_SRC_TILES) {
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-            b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                a[q] = oka ? tiles[tile_index<N, G::TC>(ra, pos)] : C{0, 0};
+                b[q] = okb ? tiles[tile_index<N, G::TC>(rb, pos)] : C{0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
+                a[q] = oka ? psi[ra * N + pos] : C{0, 0};
+                b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+            }
         }
     }
     R va[P::E], vb[P::E];
@@ -561,10 +633,32 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
                 b[q] = rotate(b[q], sb, cb);
             }
         }
-        if (flags & C2R_FWD) {
+        if (flags & C2R_DRIFT) {
+            const long long ma_ = (oka ? ra / rpm : 0) * ptab_stride, mb_ = (okb ? rb / rpm : 0) * ptab_stride;
             __syncthreads();
             fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
             fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
+            if constexpr (std::is_same<R, float>::value) {
+                const float2 *fa = ftab + ma_, *fb = ftab + mb_;
+                const float s = (float)inv_n;
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    const int pos = t + q * P::TPL;
+                    a[q] = drift_axis(fa[pos], s, a[q]);
+                    b[q] = drift_axis(fb[pos], s, b[q]);
+                }
+            } else {
+                const double2 *pa = ptab + ma_, *pb = ptab + mb_;
+                const double2 one{1.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    const int pos = t + q * P::TPL;
+                    a[q] = drift_factor(pa[pos], one, inv_n, a[q]);
+                    b[q] = drift_factor(pb[pos], one, inv_n, b[q]);
+                }
+            }
+            fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
+            fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
         }
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {

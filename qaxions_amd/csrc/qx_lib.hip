@@ -4,19 +4,24 @@
 // and Field's operators (src/field.cxx:93-402); the fused schedule of
 // qx_leapfrog merges neighbouring operators into shared HBM passes:
 //
-//   3D step:  c2r_rows[vt -> V, kick, fwd axis 2]          (K + start of D)
-//             strided fwd axis 1, psi columns -> column tiles psit
-//             strided axis 0 on psit: fwd, e^{-i dt k^2/2}/S, inv   (D)
-//             strided inv axis 1, psit -> psi columns
-//             r2c_rows[inv axis 2, store psi, density, r2c]  (end of D, P)
-//             contig fwd axis 1 on vt -> k-major vtk, strided Poisson on
-//             vtk, contig inv axis 1 vtk -> vt                         (P)
-//   2D step:  c2r_rows / strided drift / r2c_rows / contig Poisson.
-// psit = [k2/TC][i0][k1][k2%TC] (TC columns per tile) makes the leading
-// axis a column pass of stride N*TC that touches few pages per block; in
-// psi's own layout it would stride N^2 elements.  vt is plane-major,
-// [i0][k2][i1], so each row block of the real<->complex passes reads and
-// writes one plane.
+// The drift factor e^{-i dt |k|^2 / 2} is a product of one factor per axis,
+// so D = D_0 D_1 D_2 with D_a = F_a^-1 e^{-i dt k_a^2 / 2} F_a / N along axis
+// a alone: each axis' drift is one pass, and psi is back in real space after
+// every pass.  The density then needs no inverse transform of psi.
+//
+//   3D step:  c2r_rows[vt -> V, kick, drift along i2]           (P end, K, D2)
+//             strided drift along i1, psi rows -> column tiles psit   (D1)
+//             strided drift along i0 on psit (stride N*TC)            (D0)
+//             r2c_rows[psit -> density -> vt]                        (P start)
+//             contig fwd i1 vt -> k-major vtk, strided Poisson on vtk,
+//             contig inv vtk -> vt                                     (P)
+//   2D step:  c2r_rows[kick, drift along i1] / strided drift along i0 /
+//             r2c_rows[density] / contig Poisson.
+// psit = [i2/TC][i0][i1][i2%TC] (TC columns per tile) keeps the i0 pass at
+// stride N*TC, a few pages per block (psi's own i0 stride is N^2, one page
+// per element); the next c2r_rows reads psi back from psit and writes rows.
+// vt is plane-major, [i0][k2][i1], so each row block of the real<->complex
+// passes touches one plane.
 #include "../../include/qx.h"
 #include "qx_aux.hpp"
 #include "qx_engine.hpp"
@@ -129,7 +134,9 @@ struct qx_ctx {
     hipStream_t stream = nullptr;
     qx_physics ph{4.0 * M_PI, 1.0, 1, 0};
     double vmax = 0.0;
-    double2 *ptab = nullptr;   // drift phase table (N entries)
+    double2 *ptab = nullptr;   // drift phase table (N entries), then its FP32
+                               // e^{i theta} - 1 (N float2, ftab)
+    float2 *ftab = nullptr;
     double ptab_dt = std::nan("");
     // captured steady-state steps for small lattices (launch-bound)
     hipGraphExec_t gexec = nullptr;
@@ -236,6 +243,13 @@ qx_status download_r(qx_ctx *c, const void *d, double *h, long long n)
 
 // per-axis drift phase table T[i] = e^{-i dt/2 (2 pi/L)^2 n^2}, n = fold(i),
 // evaluated in FP64 on the host (libm reduces the argument exactly)
+// FP32 e^{i theta} - 1 of each table entry, the difference taken in FP64
+// and rounded once (as the device does when it multiplies factors in FP64)
+void fill_phase_m1(float2 *F, const double2 *T, long long n)
+{
+    for (long long i = 0; i < n; ++i) F[i] = float2{(float)(T[i].x - 1.0), (float)T[i].y};
+}
+
 void fill_phase_table(double2 *T, int N, double L, double dt)
 {
     const double dk = TWO_PI / L, c = -0.5 * dt * dk * dk;
@@ -253,12 +267,18 @@ bool tiled(const qx_ctx *c) { return c->dim == 3 && c->N >= c->eng->tile_cols();
 // on psi in its own layout
 qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool tiles = false)
 {
-    if (!c->ptab) TRY(ensure(c, (void **)&c->ptab, sizeof(double2) * c->N));
+    if (!c->ptab) {
+        TRY(ensure(c, (void **)&c->ptab, (sizeof(double2) + sizeof(float2)) * c->N));
+        c->ftab = (float2 *)(c->ptab + c->N);
+    }
     if (!(c->ptab_dt == dt)) {
         std::vector<double2> T(c->N);
         fill_phase_table(T.data(), c->N, c->L, dt);
+        std::vector<float2> F(c->N);
+        fill_phase_m1(F.data(), T.data(), c->N);
         HIPCHK(hipStreamSynchronize(c->stream));  // no kernel may still read the old table
         HIPCHK(hipMemcpy(c->ptab, T.data(), sizeof(double2) * c->N, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->ftab, F.data(), sizeof(float2) * c->N, hipMemcpyHostToDevice));
         c->ptab_dt = dt;
     }
     *m = MulArgs{};
@@ -270,6 +290,19 @@ qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool tiles = false)
     m->inv_s = 1.0 / (double)c->S;
     m->member_div = 1;
     return QX_OK;
+}
+// one axis' drift: factor ptab[k_a] / N on the transformed axis
+MulArgs axis_drift_args(const qx_ctx *c)
+{
+    MulArgs m{};
+    m.kind = MUL_DRIFT;
+    m.geo = GEO_AXIS;
+    m.ptab = c->ptab;
+    m.ftab = c->ftab;
+    m.inv_s = 1.0 / (double)c->N;
+    m.member_div = 1;
+    m.tcols = 1;
+    return m;
 }
 MulArgs poisson_args(const qx_ctx *c)
 {
@@ -420,44 +453,51 @@ qx_status reduce(qx_ctx *c, const void *psi, const void *V, long long n, double 
 bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
 
 // ---- the fused steady-state step, pass by pass ---------------------------
-int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 8 : 4; }
+int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 7 : 4; }
 
-// pass p of one fused KDK step acting on psi (real space, row layout), its
-// column tiles psit, vt (plane-major) and vt's k-major scratch vtk; see the
-// file header.  dm is drift_args(c, ds, &dm, tiled(c))
-void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, void *vtk, const MulArgs &dm,
+// pass p of one fused KDK step on psi (row layout), its column tiles psit,
+// vt (plane-major) and vt's k-major scratch vtk; see the file header.  am
+// is axis_drift_args(c), the drift table prepared for ds (drift_args)
+void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, void *vtk, const MulArgs &am,
                const MulArgs &pm)
 {
     MulArgs none{};
     const int N = c->N;
-    const int kick = C2R_SRC_VT | C2R_KICK | C2R_FWD;
+    const double inv_n = 1.0 / N;
     if (c->dim == 3) {
         const bool t = tiled(c);
         const long long TC = c->eng->tile_cols();
+        const int kick = C2R_SRC_VT | C2R_KICK | C2R_DRIFT | (t ? C2R_SRC_TILES : 0);
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, N, ds, nullptr, kick, nullptr, c->stream); break;
+        case 0:
+            c->eng->c2r_rows(vt, c->V, psi, c->rows, N, ds, nullptr, kick, nullptr, c->stream, psit, c->ptab,
+                             c->ftab, 0, inv_n);
+            break;
         case 1:
-            if (t) c->eng->strided_tiles(psi, psit, N, N, -1, c->stream);
-            else c->eng->strided(psi, N, N, -1, none, c->stream);
+            if (t) c->eng->strided_tiles_drift(psi, psit, N, N, am, c->stream);
+            else c->eng->strided(psi, N, N, 0, am, c->stream);
             break;
         case 2:
-            if (t) c->eng->strided(psit, N / TC, N * TC, 0, dm, c->stream);
-            else c->eng->strided(psi, 1, (long long)N * N, 0, dm, c->stream);
+            if (t) c->eng->strided(psit, N / TC, N * TC, 0, am, c->stream);
+            else c->eng->strided(psi, 1, (long long)N * N, 0, am, c->stream);
             break;
         case 3:
-            if (t) c->eng->strided_tiles(psi, psit, N, N, +1, c->stream);
-            else c->eng->strided(psi, N, N, +1, none, c->stream);
+            if (t) c->eng->r2c_rows(R2C_DENSITY_TILES, psit, nullptr, vt, c->rows, N, c->ph.pref, c->ph.rho_bar, c->stream);
+            else c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, N, c->ph.pref, c->ph.rho_bar, c->stream);
             break;
-        case 4: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, N, c->ph.pref, c->ph.rho_bar, c->stream); break;
-        case 5: c->eng->contig(vt, c->H1 * N, -1, none, c->stream, vtk, c->H1, N); break;
-        case 6: c->eng->strided(vtk, c->H1, N, 0, pm, c->stream); break;
-        case 7: c->eng->contig(vtk, c->H1 * N, +1, none, c->stream, vt, N, c->H1); break;
+        case 4: c->eng->contig(vt, c->H1 * N, -1, none, c->stream, vtk, c->H1, N); break;
+        case 5: c->eng->strided(vtk, c->H1, N, 0, pm, c->stream); break;
+        case 6: c->eng->contig(vtk, c->H1 * N, +1, none, c->stream, vt, N, c->H1); break;
         }
     } else {
+        const int kick = C2R_SRC_VT | C2R_KICK | C2R_DRIFT;
         switch (p) {
-        case 0: c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream); break;
-        case 1: c->eng->strided(psi, 1, N, 0, dm, c->stream); break;
-        case 2: c->eng->r2c_rows(R2C_INV_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 0:
+            c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, nullptr,
+                             c->ptab, c->ftab, 0, inv_n);
+            break;
+        case 1: c->eng->strided(psi, 1, N, 0, am, c->stream); break;
+        case 2: c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
     }
@@ -471,15 +511,15 @@ double pass_bytes(const qx_ctx *c, int p)
     if (c->dim == 3) {
         switch (p) {
         case 0: return H * cs + 2 * S * cs;   // read vt, rw psi
-        case 4: return 2 * S * cs + H * cs;   // rw psi, write vt
-        case 1: case 2: case 3: return 2 * S * cs;
+        case 1: case 2: return 2 * S * cs;
+        case 3: return S * cs + H * cs;       // read psi, write vt
         default: return 2 * H * cs;
         }
     }
     switch (p) {
     case 0: return H * cs + 2 * S * cs;
     case 1: return 2 * S * cs;
-    case 2: return 2 * S * cs + H * cs;
+    case 2: return S * cs + H * cs;
     default: return 2 * H * cs;
     }
 }
@@ -790,7 +830,7 @@ static bool same_physics(const qx_physics &a, const qx_physics &b)
            a.rho_bar_mode == b.rho_bar_mode;
 }
 
-static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, const MulArgs &pm)
+static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &am, const MulArgs &pm)
 {
     const int np = step_pass_count(c);
     int done = 0;
@@ -801,7 +841,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, co
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < GRAPH_STEPS; ++i)
-                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, dm, pm);
+                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, am, pm);
             HIPCHK(hipStreamEndCapture(c->stream, &g));
             const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -812,7 +852,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &dm, co
         for (; done + GRAPH_STEPS <= n; done += GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     }
     for (; done < n; ++done)
-        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, dm, pm);
+        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, am, pm);
     return QX_OK;
 }
 
@@ -828,20 +868,25 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
     }
     const MulArgs pm = poisson_args(c);
     MulArgs dm;
-    TRY(drift_args(c, ds, &dm, tiled(c)));
-    if (tiled(c)) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+    TRY(drift_args(c, ds, &dm));   // the phase table for ds
+    const MulArgs am = axis_drift_args(c);
+    const bool t = tiled(c);
+    if (t) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
     const int np = step_pass_count(c);
+    const long long rpm = c->dim == 3 ? c->N : c->rows;
     // the opening kick reads the stored V; later kicks the potential's spectrum
-    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, c->N, k_first, nullptr, C2R_KICK | C2R_FWD, nullptr,
-                     c->stream);
-    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, dm, pm);
-    TRY(steady_steps(c, nsteps - 1, ds, dm, pm));
-    // last step's c2r: materialise V and max|V|, closing half kick
+    c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, rpm, k_first, nullptr, C2R_KICK | C2R_DRIFT, nullptr,
+                     c->stream, nullptr, c->ptab, c->ftab, 0, 1.0 / c->N);
+    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, am, pm);
+    TRY(steady_steps(c, nsteps - 1, ds, am, pm));
+    // last step's c2r: materialise V and max|V|, closing half kick; psi
+    // comes back from its tiles into the row layout
     if (wait) TRY(reset_vmax(c));
-    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, c->N, 0.5 * ds, nullptr,
-                     C2R_SRC_VT | C2R_STORE_V | (wait ? C2R_VMAX : 0) | (last_half ? C2R_KICK : 0),
-                     c->vmax_bits, c->stream);
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, rpm, 0.5 * ds, nullptr,
+                     C2R_SRC_VT | C2R_STORE_V | (wait ? C2R_VMAX : 0) | (last_half ? C2R_KICK : 0) |
+                         (t ? C2R_SRC_TILES : 0),
+                     c->vmax_bits, c->stream, c->psit);
     return wait ? read_vmax(c) : QX_OK;
 }
 
@@ -861,10 +906,9 @@ static bool pass_slot(const qx_ctx *c, int p, int *kind, int *cls)
     if (c->dim == 3) {
         const bool t = tiled(c);
         switch (p) {
-        case 1: *kind = 0; *cls = N > 4096 ? 1 : 0; return !t;   // fwd axis 1 (tile passes: fixed width)
-        case 2: *kind = 2; *cls = (t ? N * c->eng->tile_cols() : N * N) > 4096 ? 1 : 0; return true;  // drift
-        case 3: *kind = 1; *cls = N > 4096 ? 1 : 0; return !t;   // inv axis 1
-        case 6: *kind = 3; *cls = N > 4096 ? 1 : 0; return true;  // vt Poisson
+        case 1: *kind = 2; *cls = N > 4096 ? 1 : 0; return !t;   // drift along i1 (tile pass: fixed width)
+        case 2: *kind = 2; *cls = (t ? N * c->eng->tile_cols() : N * N) > 4096 ? 1 : 0; return true;  // along i0
+        case 5: *kind = 3; *cls = N > 4096 ? 1 : 0; return true;  // vt Poisson
         default: return false;
         }
     }
@@ -911,9 +955,9 @@ qx_status qx_set_tile_variants(qx_ctx *c, const int *variants)
 
 const char *qx_step_pass_name(const qx_ctx *c, int p)
 {
-    static const char *n3[] = {"c2r_kick_fwd2", "fwd1_tiles", "drift0", "inv1_tiles", "inv2_density_r2c",
+    static const char *n3[] = {"c2r_kick_drift2", "drift1_tiles", "drift0", "density_r2c",
                                "vt_fwd1", "vt_poisson0", "vt_inv1"};
-    static const char *n2[] = {"c2r_kick_fwd1", "drift0", "inv1_density_r2c", "vt_poisson0"};
+    static const char *n2[] = {"c2r_kick_drift1", "drift0", "density_r2c", "vt_poisson0"};
     if (!valid_ctx(c) || p < 0 || p >= step_pass_count(c)) return "";
     return c->dim == 3 ? n3[p] : n2[p];
 }
@@ -929,17 +973,20 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     HIPCHK(hipMemcpyAsync(c->aux_vt, c->vt, (size_t)c->H * c->csz, hipMemcpyDeviceToDevice, c->stream));
     const MulArgs pm = poisson_args(c);
     MulArgs dm;
-    TRY(drift_args(c, ds, &dm, tiled(c)));
+    TRY(drift_args(c, ds, &dm));
+    const MulArgs am = axis_drift_args(c);
     if (tiled(c)) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
+    // pass 0 of a tiled lattice reads psi from its tiles: fill them first
+    if (tiled(c)) step_pass(c, 1, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     qx_status st = QX_OK;
     for (int p = 0; p < np && st == QX_OK; ++p) {
-        step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, dm, pm);  // warm
+        step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);  // warm
         if (hipEventRecord(e0, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
-        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, dm, pm);
+        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
         if (hipEventRecord(e1, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
         float t = 0.f;
         if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
@@ -1152,7 +1199,8 @@ struct EnsClass {
     int N = 0, B = 0;
     long long S = 0, rows = 0, H1 = 0, H = 0;
     void *psi = nullptr, *V = nullptr, *vt = nullptr;
-    double2 *d_ptab = nullptr;                                 // per member phase tables [B][N]
+    double2 *d_ptab = nullptr;                                 // per member phase tables [B][N],
+                                                               // then FP32 e^{i theta} - 1 [B][N]
     double *d_pfac = nullptr;                                  // per member
     double *d_kick_first = nullptr, *d_kick_mid = nullptr, *d_kick_last = nullptr;
     std::unique_ptr<Engine> eng;
@@ -1160,6 +1208,8 @@ struct EnsClass {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
 };
+
+float2 *ens_ftab(const EnsClass &k) { return (float2 *)(k.d_ptab + (size_t)k.B * k.N); }
 
 }  // namespace
 
@@ -1209,6 +1259,9 @@ qx_status ens_set_ds(qx_ens *e, const double *ds)
             kf[s] = 0.5 * ds[m];
         }
         HIPCHK(hipMemcpy(k.d_ptab, tab.data(), sizeof(double2) * tab.size(), hipMemcpyHostToDevice));
+        std::vector<float2> ftab(tab.size());
+        fill_phase_m1(ftab.data(), tab.data(), (long long)tab.size());
+        HIPCHK(hipMemcpy(ens_ftab(k), ftab.data(), sizeof(float2) * ftab.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(k.d_pfac, pf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(k.d_kick_mid, km.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(k.d_kick_last, kl.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
@@ -1221,9 +1274,10 @@ MulArgs ens_drift_args(const EnsClass &k)
 {
     MulArgs m{};
     m.kind = MUL_DRIFT;
-    m.geo = GEO_PSI2;
-    m.inv_s = 1.0 / (double)k.S;
+    m.geo = GEO_AXIS;        // drift along i0; i1's is in c2r_rows
+    m.inv_s = 1.0 / (double)k.N;
     m.ptab = k.d_ptab;
+    m.ftab = ens_ftab(k);
     m.ptab_stride = k.N;
     m.member_div = 1;
     m.tcols = 1;
@@ -1244,10 +1298,10 @@ MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
 void ens_class_step(qx_ens *e, EnsClass &k, hipStream_t s)
 {
     const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
-    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_FWD,
-                    nullptr, s);
+    k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_DRIFT,
+                    nullptr, s, nullptr, k.d_ptab, ens_ftab(k), k.N, 1.0 / k.N);
     k.eng->strided(k.psi, k.B, k.N, 0, dm, s);
-    k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
+    k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
     k.eng->contig(k.vt, k.B * k.H1, 0, pm, s);
 }
 
@@ -1360,7 +1414,7 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
                          (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
         const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
                              (size_t)(k.B * k.H) * e->csz, (size_t)k.N * e->csz,
-                             sizeof(double2) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
+                             (sizeof(double2) + sizeof(float2)) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
                              sizeof(double) * k.B, sizeof(double) * k.B};
         for (int b = 0; b < 9; ++b)
             if (hipMalloc(bufs[b], sz[b]) != hipSuccess) {
@@ -1524,9 +1578,10 @@ qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_hal
         for (int s = 0; s < k.B; ++s) kf[s] = (first_half ? 0.5 : 1.0) * ds[k.members[s]];
         HIPCHK(hipMemcpy(k.d_kick_first, kf.data(), sizeof(double) * k.B, hipMemcpyHostToDevice));
         const MulArgs dm = ens_drift_args(k), pm = ens_poisson_args(e, k);
-        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_FWD, nullptr, e->stream);
+        k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_DRIFT, nullptr,
+                        e->stream, nullptr, k.d_ptab, ens_ftab(k), k.N, 1.0 / k.N);
         k.eng->strided(k.psi, k.B, k.N, 0, dm, e->stream);
-        k.eng->r2c_rows(R2C_INV_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
+        k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
         k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
     }
     TRY(ens_steady_steps(e, nsteps - 1));
