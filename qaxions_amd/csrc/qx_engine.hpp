@@ -98,8 +98,7 @@ template <int N, typename R> struct EngineT final : Engine {
     void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
                              hipStream_t s) override
     {
-        // 16-column tiles take two columns per thread (512 threads)
-        constexpr int TC = G::TC, CPT = (TC >= 16 && TC % 2 == 0) ? 2 : 1;
+        constexpr int TC = G::TC, CPT = 1;
         const long long ntiles = inner / TC;
         const unsigned blocks = (unsigned)(outer * ntiles);
         strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
