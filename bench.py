@@ -183,7 +183,9 @@ def main():
                      "bytes_per_launch": pbytes[dom], "ms_per_launch": pms[dom]},
         "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_site": B_site,
                           "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                          "unit": "GB/s", "frac": step_achieved / HBM_PEAK},
+                          "unit": "GB/s", "frac": step_achieved / HBM_PEAK,
+                          "physical_bytes_per_site": sum(pbytes) / S,
+                          "physical_frac": sum(pbytes) / (ms_step * 1e-3) / HBM_PEAK},
         "passes": {n_: {"us": t_ * 1e3, "GBps": b_ / t_ / 1e6} for n_, t_, b_ in zip(names, pms, pbytes)},
         "mass_drift": abs(m1 - m0) / m0,
         "wall_ms_per_step": wall * 1e3 / args.steps,
