@@ -206,4 +206,88 @@ __device__ __forceinline__ void fft_line(C (&v)[Plan<N>::E], int t, C *line, int
     stage_loop<N, 0, DIR, L>(v, t, line, tc, tw);
 }
 
+// Two lines transformed in lockstep (same direction): each stage's
+// butterflies of both lines, one synchronisation per exchange for both, two
+// independent dependency chains for the scheduler.  line_a / line_b are the
+// two lines' LDS areas.
+template <int N, int S, int DIR, typename L, typename C>
+__device__ __forceinline__ void fft_stage2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>::E], int t, C *line_a,
+                                           C *line_b, int tc, const C *tw)
+{
+    using P = Plan<N>;
+    constexpr int E = P::E, TPL = P::TPL;
+    constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
+    constexpr bool last = (S == P::NST - 1);
+    constexpr int TWS = N / (Ns * R);
+    C oa[E], ob[E];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int j = t + m * TPL;
+        const int k = j & (Ns - 1);
+        C ua[R], ub[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            ua[r] = va[m + r * NB];
+            ub[r] = vb[m + r * NB];
+        }
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C w = tw[r * k * TWS];
+                if (DIR > 0) w.y = -w.y;
+                ua[r] = cmul(ua[r], w);
+                ub[r] = cmul(ub[r], w);
+            }
+        }
+        Dft<R, DIR, C>::run(ua);
+        Dft<R, DIR, C>::run(ub);
+        if (last) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                oa[m + r * NB] = ua[r];
+                ob[m + r * NB] = ub[r];
+            }
+        } else {
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                line_a[L::at(base + r * Ns, tc)] = ua[r];
+                line_b[L::at(base + r * Ns, tc)] = ub[r];
+            }
+        }
+    }
+    if (last) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            va[q] = oa[q];
+            vb[q] = ob[q];
+        }
+    } else {
+        line_sync<N, L>();
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            va[q] = line_a[L::at(t + q * TPL, tc)];
+            vb[q] = line_b[L::at(t + q * TPL, tc)];
+        }
+        line_sync<N, L>();
+    }
+}
+
+template <int N, int S, int DIR, typename L, typename C>
+__device__ __forceinline__ void stage_loop2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>::E], int t, C *line_a,
+                                            C *line_b, int tc, const C *tw)
+{
+    if constexpr (S < Plan<N>::NST) {
+        fft_stage2<N, S, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
+        stage_loop2<N, S + 1, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
+    }
+}
+
+template <int N, int DIR, typename L, typename C>
+__device__ __forceinline__ void fft_line2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>::E], int t, C *line_a,
+                                          C *line_b, int tc, const C *tw)
+{
+    stage_loop2<N, 0, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
+}
+
 }  // namespace qx

@@ -51,9 +51,16 @@ template <int N, typename R> struct Geom {
     static constexpr int NP = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
     static constexpr int TL = 2 * NP;                 // rows per block
     static constexpr int ROW_THREADS = NP * P::TPL;
-    static constexpr int XCH = NP * P::PADDED;        // exchange slots
+    // c2r_rows transforms a pair's two psi rows in lockstep, each with its
+    // own exchange line, where two lines per pair fit in LDS beside the
+    // twiddles (else one after the other through one line)
+    static constexpr bool TWO_LINES =
+        (long long)(2 * NP * P::PADDED + N) * (long long)sizeof(C) + 256 <= 163840;
+    static constexpr int XCH1 = NP * P::PADDED;                        // one line per pair
+    static constexpr int XCH = (TWO_LINES ? 2 : 1) * XCH1;             // c2r_rows' exchange slots
     static constexpr int STG = (N / 2 + 1) * TL;      // staging slots (half spectrum)
-    static constexpr int ROW_LDS = XCH > STG ? XCH : STG;
+    static constexpr int ROW_LDS = XCH > STG ? XCH : STG;       // c2r_rows
+    static constexpr int ROW_LDS1 = XCH1 > STG ? XCH1 : STG;    // r2c_rows
 };
 
 // pointwise factor in fused passes
@@ -414,7 +421,7 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     using P = Plan<N>;
     using G = Geom<N, R>;
     constexpr int TL = G::TL, H1 = N / 2 + 1;
-    __shared__ C lds[G::ROW_LDS];
+    __shared__ C lds[G::ROW_LDS1];
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
@@ -635,9 +642,14 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         }
         if (flags & C2R_DRIFT) {
             const long long ma_ = (oka ? ra / rpm : 0) * ptab_stride, mb_ = (okb ? rb / rpm : 0) * ptab_stride;
+            C *mine_b = lds + (G::NP + p) * P::PADDED;
             __syncthreads();
-            fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
-            fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
+            if constexpr (G::TWO_LINES) {
+                fft_line2<N, -1, LdsContig>(a, b, t, mine, mine_b, 1, tw);
+            } else {
+                fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
+                fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
+            }
             if constexpr (std::is_same<R, float>::value) {
                 const float2 *fa = ftab + ma_, *fb = ftab + mb_;
                 const float s = (float)inv_n;
@@ -657,8 +669,12 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
                     b[q] = drift_factor(pb[pos], one, inv_n, b[q]);
                 }
             }
-            fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
-            fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
+            if constexpr (G::TWO_LINES) {
+                fft_line2<N, +1, LdsContig>(a, b, t, mine, mine_b, 1, tw);
+            } else {
+                fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
+                fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
+            }
         }
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
