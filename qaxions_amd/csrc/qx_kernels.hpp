@@ -58,7 +58,11 @@ template <int N, typename R> struct Geom {
         (long long)(2 * NP * P::PADDED + N) * (long long)sizeof(C) + 256 <= 163840;
     static constexpr int XCH1 = NP * P::PADDED;                        // one line per pair
     static constexpr int XCH = (TWO_LINES ? 2 : 1) * XCH1;             // c2r_rows' exchange slots
-    static constexpr int STG = (N / 2 + 1) * TL;      // staging slots (half spectrum)
+    // staging of the half spectrum, [k][row] at pitch TL + 1 (TL >= 8): a
+    // wave reading one row at 64 consecutive k then spreads over all banks
+    // (at pitch TL = 16, 128 bytes, every lane hits the same bank pair)
+    static constexpr int STP = TL >= 8 ? TL + 1 : TL;
+    static constexpr int STG = (N / 2 + 1) * STP;     // staging slots
     static constexpr int ROW_LDS = XCH > STG ? XCH : STG;       // c2r_rows
     static constexpr int ROW_LDS1 = XCH1 > STG ? XCH1 : STG;    // r2c_rows
 };
@@ -489,19 +493,19 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
         }
     }
     __syncthreads();
-    C *stg = lds;  // staging aliases the exchange area: stg[k*TL + l]
+    C *stg = lds;  // staging aliases the exchange area: stg[k*STP + l]
 #pragma unroll
     for (int q = 0; q <= P::E / 2; ++q) {
         const int k = t + q * P::TPL;
         if (k <= N / 2) {
-            stg[k * TL + 2 * p] = oa[q];
-            stg[k * TL + 2 * p + 1] = ob[q];
+            stg[k * G::STP + 2 * p] = oa[q];
+            stg[k * G::STP + 2 * p + 1] = ob[q];
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
         const int k = i / TL, l = i % TL;
-        if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[i];
+        if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[k * G::STP + l];
     }
 }
 
@@ -577,7 +581,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const int i = threadIdx.x + j * G::ROW_THREADS;
-            if (i < H1 * TL) stg[i] = tmp[j];
+            if (i < H1 * TL) stg[(i / TL) * G::STP + i % TL] = tmp[j];
         }
         __syncthreads();
         C z[P::E];
@@ -586,7 +590,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             const int j = t + q * P::TPL;
             const bool lo = j <= N / 2;
             const int k = lo ? j : N - j;
-            C xa = stg[k * TL + 2 * p], xb = stg[k * TL + 2 * p + 1];
+            C xa = stg[k * G::STP + 2 * p], xb = stg[k * G::STP + 2 * p + 1];
             if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
             // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
             z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
