@@ -253,6 +253,54 @@ def test_plain_method_256sq_100_steps(qx, prec, tol):
     assert np.all(np.abs(P - Po) <= rtol * Po + afl * Po.max())
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_config1_blobs_vs_oracle(qx, prec, tol):
+    """The 3D soliton-blob workload (32 seeded Gaussian blobs on background
+    0.1, L = 20, dt = 5e-4, 200 KDK steps, mean density subtracted) at 64^3,
+    where the FP64 oracle runs in seconds: psi against the oracle, and the
+    oracle's own mass and energy conservation reproduced."""
+    N, L, dt, steps = 64, 20.0, 5e-4, 200
+    tab = o.blob_table(32, 7, L)
+    psi0 = o.ic_blobs(3, N, L, tab, 0.1)
+    V0, _, _ = o.update_potential(psi0, L, 1.0, 1.0, 0, 0)
+    want, Vw = o.leapfrog(psi0, V0, L, dt, steps, 1.0, 1.0, 0, 0)
+    lat = qx.Lattice(3, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+    lat.psi = psi0
+    lat.update_potential()
+    e0 = lat.energy()
+    lat.leapfrog(dt, steps)
+    err = rel_l2(lat.psi, want)
+    print(f"psi rel L2 after {steps} steps ({prec}): {err:.3e}")
+    assert err <= tol
+    assert rel_l2(lat.V, Vw) <= (1e-12 if prec == "fp64" else 2e-5)
+    e1 = lat.energy()
+    assert abs(e1[0] - e0[0]) / e0[0] < (1e-13 if prec == "fp64" else 1e-5)
+
+
+def test_config1_256cube_fp32_against_fp64(qx):
+    """The same workload at its full 256^3 size: the FP32 run against the
+    FP64 run (pinned to the oracle above), mass and energy drift over 200
+    steps.  FP64 energy changes by the method's own 3e-9; FP32 adds round-off."""
+    N, L, dt, steps = 256, 20.0, 5e-4, 200
+    tab = o.blob_table(32, 7, L)
+    out, drift = {}, {}
+    for prec in ("fp64", "fp32"):
+        lat = qx.Lattice(3, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.ic_blobs(tab, 0.1)
+        lat.update_potential()
+        e0 = lat.energy()
+        lat.leapfrog(dt, steps)
+        e1 = lat.energy()
+        out[prec] = lat.psi
+        drift[prec] = (abs(e1[0] - e0[0]) / e0[0], abs((e1[1] + e1[2]) - (e0[1] + e0[2])) / abs(e0[1] + e0[2]))
+        lat.close()
+    err = rel_l2(out["fp32"], out["fp64"])
+    print(f"256^3 fp32 vs fp64 after {steps} steps: {err:.3e}; drifts {drift}")
+    assert err < 2e-5
+    assert drift["fp64"][0] < 1e-12 and drift["fp64"][1] < 1e-8
+    assert drift["fp32"][0] < 1e-5 and drift["fp32"][1] < 1e-4
+
+
 # ------------------------------------------------------------------ initial conditions
 @pytest.mark.parametrize("dim,N", [(2, 64), (3, 16)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -270,7 +318,7 @@ def test_device_ics(qx, dim, N, prec):
 
 
 # ------------------------------------------------------------------ larger lattices
-@pytest.mark.parametrize("dim,N", [(3, 256), (2, 2048)])
+@pytest.mark.parametrize("dim,N", [(3, 256), (2, 2048), (3, 1024)])
 def test_large_lattice_invariants(qx, dim, N):
     """Size-independent properties at large N: round trip of the transform,
     mass conservation through fused steps, determinism."""
