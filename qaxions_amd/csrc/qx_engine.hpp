@@ -18,14 +18,9 @@ struct Engine {
                         void *out = nullptr, long long rm_m = 1, long long rm_a = 1) = 0;
     virtual void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                          hipStream_t s) = 0;
-    // out-of-place column pass between data and its column tiles (see
-    // strided_fft): forward, op < 0, data columns -> tiles; inverse, op > 0,
-    // tiles -> data columns.  Tiles are tile_cols() wide; inner must be a
-    // multiple of it
-    virtual void strided_tiles(void *data, void *tiles, long long outer, long long inner, int op,
-                               hipStream_t s) = 0;
     // drift along the strided axis (fwd, the axis' factor, inv), data columns
-    // -> tiles
+    // -> column tiles (see strided_fft), tile_cols() wide; inner must be a
+    // multiple of it
     virtual void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner,
                                      const MulArgs &m, hipStream_t s) = 0;
     virtual int tile_cols() const = 0;
@@ -55,18 +50,6 @@ template <int N, typename R> struct EngineT final : Engine {
     using G = Geom<N, R>;
     using P = Plan<N>;
 
-    // persistent grids (row kernels): as many blocks as fit on the device
-    template <typename K> static long long resident_blocks(K kernel, int threads)
-    {
-        int dev = 0, cus = 0, per = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0);
-        if (cus < 1) cus = 1;
-        if (per < 1) per = 1;
-        return (long long)cus * per;
-    }
-
     void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s, void *out,
                 long long rm_m, long long rm_a) override
     {
@@ -80,21 +63,6 @@ template <int N, typename R> struct EngineT final : Engine {
 
     // the tile width is the full column tile, whatever the variant of the
     // plain passes: the tile layout must not depend on tuning
-    void strided_tiles(void *data, void *tiles, long long outer, long long inner, int op, hipStream_t s) override
-    {
-        constexpr int TC = G::TC;
-        const long long ntiles = inner / TC;
-        const unsigned blocks = (unsigned)(outer * ntiles);
-        const int threads = TC * P::TPL;
-        MulArgs none{};
-        none.member_div = 1;
-        C *d = (C *)data, *l = (C *)tiles;
-        const C *t = (const C *)tw;
-        if (op < 0)
-            strided_fft<N, R, OP_FWD, TC, XM_STORE_TILES><<<blocks, threads, 0, s>>>(d, inner, ntiles, none, t, l, outer);
-        else
-            strided_fft<N, R, OP_INV, TC, XM_LOAD_TILES><<<blocks, threads, 0, s>>>(d, inner, ntiles, none, t, l, outer);
-    }
     void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
                              hipStream_t s) override
     {
@@ -116,9 +84,8 @@ template <int N, typename R> struct EngineT final : Engine {
         const C *t = (const C *)tw;
         if (op < 0) strided_fft<N, R, OP_FWD, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
         else if (op > 0) strided_fft<N, R, OP_INV, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
-        else if (m.kind == MUL_DRIFT && m.geo == GEO_AXIS)
+        else if (m.kind == MUL_DRIFT)
             strided_fft<N, R, OP_DRIFT_AXIS, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
-        else if (m.kind == MUL_DRIFT) strided_fft<N, R, OP_DRIFT, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
         else strided_fft<N, R, OP_POISSON, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
     }
     void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
@@ -139,8 +106,6 @@ template <int N, typename R> struct EngineT final : Engine {
             qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else if (mode == R2C_DENSITY)
             qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
-        else if (mode == R2C_INV_DENSITY)
-            qx::r2c_rows<N, R, R2C_INV_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else
             qx::r2c_rows<N, R, R2C_DENSITY_TILES><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
     }

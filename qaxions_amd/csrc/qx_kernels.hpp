@@ -72,19 +72,16 @@ enum { MUL_NONE = 0, MUL_DRIFT = 1, MUL_POISSON = 2 };
 // pass operations (template argument): plain transforms, or forward,
 // pointwise factor and inverse in one pass; separate instantiations keep the
 // factor code out of the plain passes and name each pass in profiles
-enum { OP_FWD = -1, OP_INV = 1, OP_DRIFT = 0, OP_POISSON = 2, OP_DRIFT_AXIS = 3 };
-// OP_DRIFT_AXIS: one axis' drift (GEO_AXIS), the factor of the line position
-// only; FP32 lattices read it from the precomputed FP32 table ftab
-// geometry of the multiplied pass: how the 3 (or 2) integer frequencies of
-// an element follow from (line position p, column/line index c, outer o)
+enum { OP_FWD = -1, OP_INV = 1, OP_POISSON = 2, OP_DRIFT_AXIS = 3 };
+// OP_DRIFT_AXIS: one axis' drift, the factor of the line position only
+// (e^{-i dt k_a^2 / 2} / N); FP32 lattices read it from the precomputed FP32
+// table ftab
+// geometry of the multiplied pass: how the integer frequencies of an element
+// follow from (line position p, column/line index c, outer o)
 enum {
-    GEO_PSI3 = 0,  // psi, axis 0 strided, c = i1*N + i2 (already transformed)
-    GEO_PSI2 = 1,  // psi 2D, axis 0 strided, c = k1
     GEO_VT3 = 2,   // vt 3D, axis i0 strided inside plane o = k2, c = k1
     GEO_VT2 = 3,   // vt 2D, contiguous lines along i0, line index = k1
-    GEO_PSIT3 = 4, // 3D psi in column tiles [k2 / TT][i0][k1][k2 % TT] (TT = tcols):
-                   // axis i0 strided, o = k2 / TT, c = k1 * TT + k2 % TT
-    GEO_AXIS = 5,  // one axis' own factor: the table entry of the line position
+    GEO_AXIS = 5,  // one axis' drift: the table entry of the line position
 };
 
 struct MulArgs {
@@ -105,7 +102,6 @@ struct MulArgs {
     // line index) / member_div; null for a single lattice
     const double *member_par;
     long long member_div;
-    int tcols;       // GEO_PSIT3: columns per tile
 };
 
 // the multiply's constants for one member of an ensemble
@@ -192,25 +188,16 @@ __device__ __forceinline__ C poisson_factor(const MulArgs &a, int m, C x)
     return C{x.x * f, x.y * f};
 }
 
-// drift factor e^{i theta}/S: the table entry of the line position times the
-// column factor, both FP64.  Applied as (x + x (e^{i theta} - 1)) / S with
-// e^{i theta} - 1 formed in FP64, so the FP32 result rounds once, unbiased
-// (S is a power of two).
-__device__ __forceinline__ float2 drift_factor(const double2 &tp, const double2 &cf, double inv_s, float2 x)
-{
-    const double2 f = zmul(tp, cf);
-    const float fr = (float)(f.x - 1.0), fi = (float)f.y;
-    const float tr = fmaf(x.x, fr, -x.y * fi), ti = fmaf(x.x, fi, x.y * fr);
-    const float s = (float)inv_s;
-    return float2{(x.x + tr) * s, (x.y + ti) * s};
-}
-// one axis' drift factor in FP32 from its precomputed e^{i theta} - 1 (fm1):
-// the same arithmetic as drift_factor with a unit column factor
+// one axis' drift factor e^{i theta} / N in FP32, from e^{i theta} - 1
+// formed in FP64 and rounded once (fm1): applied as (x + x (e^{i theta} - 1))
+// / N, so the result rounds once, unbiased (N is a power of two); rounding
+// e^{i theta} itself near 1 would bias |psi| and drift the mass
 __device__ __forceinline__ float2 drift_axis(float2 fm1, float s, float2 x)
 {
     const float tr = fmaf(x.x, fm1.x, -x.y * fm1.y), ti = fmaf(x.x, fm1.y, x.y * fm1.x);
     return float2{(x.x + tr) * s, (x.y + ti) * s};
 }
+// FP64: the table entry tp times the column factor cf, times inv_s
 __device__ __forceinline__ double2 drift_factor(const double2 &tp, const double2 &cf, double inv_s, double2 x)
 {
     const double2 f = zmul(tp, cf);
@@ -292,22 +279,20 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
 // many blocks share a CU and overlap their loads with each other's
 // transforms; the variant is picked per pass by measurement.
 //
-// Out-of-place tile modes (3D psi, whose rows have been transformed along
-// the last axis): a block's N x TC tile of columns (col0..col0+TC-1 of outer
-// index o) lives contiguously in `tiles` at ((col0/TC)*outer + o)*N*TC, as
-// [pos][col % TC].  XM_STORE_TILES transforms the columns of data and writes
-// the tile; XM_LOAD_TILES reads the tile, transforms it and writes columns of
-// data.  With data = psi[i0][i1][k2] (o = i0, col = k2) the tiles form
-// [k2/TC][i0][k1][k2%TC]: the i0 axis is a column pass of stride N*TC over
-// them (GEO_PSIT3), 128-byte segments in 16 pages per block, where psi's own
-// i0 axis (stride N^2) would take one page per element.
+// Out-of-place tile mode (3D psi): a block's N x TC tile of columns
+// (col0..col0+TC-1 of outer index o) is written contiguously to `tiles` at
+// ((col0/TC)*outer + o)*N*TC, as [pos][col % TC].  With data =
+// psi[i0][i1][i2] (o = i0, col = i2, the pass along i1) the tiles form
+// [i2/TC][i0][i1][i2%TC] (see tile_index): their i0 axis is a column pass of
+// stride N*TC, 128-byte segments in 16 pages per block, where psi's own i0
+// axis (stride N^2) would take one page per element.
 // ---------------------------------------------------------------------------
-enum { XM_NONE = 0, XM_STORE_TILES = 1, XM_LOAD_TILES = 2 };
+enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 
 // CPT columns per thread: a block of TC/CPT thread columns transforms its
 // CPT columns one after another through one LDS area (CPT = 2 keeps a
 // 16-column tile's 128-byte row segments with 512 threads per block).
-template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1>  // OP_FWD, OP_INV, OP_DRIFT(_AXIS) or OP_POISSON
+template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1>  // OP_FWD, OP_INV, OP_DRIFT_AXIS or OP_POISSON
 __global__ void __launch_bounds__(((TC / CPT) * Plan<N>::TPL))
 strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
             MulArgs mul, const typename cplx<R>::T *__restrict__ twg,
@@ -321,7 +306,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     // the drift's phase table is staged in LDS when small (N <= 1024); an
     // FP32 single-axis drift stages its FP32 table
     constexpr bool FAX = OP == OP_DRIFT_AXIS && std::is_same<R, float>::value;
-    constexpr bool PT_LDS = (OP == OP_DRIFT || OP == OP_DRIFT_AXIS) && N <= 1024;
+    constexpr bool PT_LDS = OP == OP_DRIFT_AXIS && N <= 1024;
     __shared__ double2 ptab_lds[PT_LDS ? (FAX ? N / 2 : N) : 1];
     load_tw<N>(tw, twg);
     const int c = threadIdx.x % TW, t = threadIdx.x / TW;
@@ -346,10 +331,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     for (int j = 0; j < CPT; ++j) {
         const bool ok = col0 + c + j * TW < inner;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            if constexpr (XM == XM_LOAD_TILES) v[j][q] = ok ? tile0[(t + q * P::TPL) * TC + j * TW] : C{0, 0};
-            else v[j][q] = ok ? base0[(long long)(t + q * P::TPL) * inner + j * TW] : C{0, 0};
-        }
+        for (int q = 0; q < P::E; ++q) v[j][q] = ok ? base0[(long long)(t + q * P::TPL) * inner + j * TW] : C{0, 0};
     }
     __syncthreads();
 #pragma unroll
@@ -371,14 +353,6 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
                 const double2 one{1.0, 0.0};
 #pragma unroll
                 for (int q = 0; q < P::E; ++q) v[j][q] = drift_factor(ptab[t + q * P::TPL], one, ma.inv_s, v[j][q]);
-            } else if constexpr (OP == OP_DRIFT) {
-                double2 cf;
-                if (mul.geo == GEO_PSI3) cf = zmul(ptab[cc / N], ptab[cc % N]);
-                else if (mul.geo == GEO_PSIT3)
-                    cf = zmul(ptab[cc / mul.tcols], ptab[(int)o * mul.tcols + cc % mul.tcols]);
-                else cf = ptab[cc];
-#pragma unroll
-                for (int q = 0; q < P::E; ++q) v[j][q] = drift_factor(ptab[t + q * P::TPL], cf, ma.inv_s, v[j][q]);
             } else {  // GEO_VT3 Poisson
                 const int a = foldi(cc, N);
                 const int mc = a * a + (int)(o * o);
@@ -409,8 +383,6 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
 enum {
     R2C_REAL = 0,      // input: real rows (vin)
     R2C_DENSITY = 1,   // input: psi rows, rho = pref (|psi|^2 - rho_bar)
-    R2C_INV_DENSITY = 2,  // input: psi rows transformed along the last axis;
-                          // inverse-transform, store psi, then as DENSITY
     R2C_DENSITY_TILES = 3,  // as DENSITY, psi read from a 3D lattice's column
                             // tiles (the `psi` argument; see tile_index)
 };
@@ -456,16 +428,6 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
             }
         }
         __syncthreads();
-        if (MODE == R2C_INV_DENSITY) {
-            fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
-            fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                if (oka) psi[ra * N + pos] = a[q];
-                if (okb) psi[rb * N + pos] = b[q];
-            }
-        }
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             xa[q] = pref * (a[q].x * a[q].x + a[q].y * a[q].y - rho_bar);
@@ -565,7 +527,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             }
         }
     }
-    R va[P::E], vb[P::E];
+    R va[P::E] = {}, vb[P::E] = {};
     if (flags & C2R_SRC_VT) {
         // stage [k][l] of the block's TL rows: all loads issued before the
         // first LDS store
@@ -599,7 +561,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
 #pragma unroll
         for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
-    } else {
+    } else if (flags & (C2R_STORE_V | C2R_VMAX | C2R_KICK)) {
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int pos = t + q * P::TPL;

@@ -263,9 +263,9 @@ void fill_phase_table(double2 *T, int N, double L, double dt)
 // 3D lattices whose leading-axis drift runs on psi's column tiles
 bool tiled(const qx_ctx *c) { return c->dim == 3 && c->N >= c->eng->tile_cols(); }
 
-// tiles: the factor applied on the column tiles psit (tiled(c)) rather than
-// on psi in its own layout
-qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool tiles = false)
+// the per-axis phase table for step dt (FP64 entries and their FP32
+// e^{i theta} - 1), uploaded when dt changes
+qx_status load_phase_table(qx_ctx *c, double dt)
 {
     if (!c->ptab) {
         TRY(ensure(c, (void **)&c->ptab, (sizeof(double2) + sizeof(float2)) * c->N));
@@ -281,14 +281,6 @@ qx_status drift_args(qx_ctx *c, double dt, MulArgs *m, bool tiles = false)
         HIPCHK(hipMemcpy(c->ftab, F.data(), sizeof(float2) * c->N, hipMemcpyHostToDevice));
         c->ptab_dt = dt;
     }
-    *m = MulArgs{};
-    m->kind = MUL_DRIFT;
-    m->geo = tiles ? GEO_PSIT3 : (c->dim == 3 ? GEO_PSI3 : GEO_PSI2);
-    m->tcols = tiles ? c->eng->tile_cols() : 1;
-    m->ptab = c->ptab;
-    m->ptab_stride = 0;
-    m->inv_s = 1.0 / (double)c->S;
-    m->member_div = 1;
     return QX_OK;
 }
 // one axis' drift: factor ptab[k_a] / N on the transformed axis
@@ -301,7 +293,6 @@ MulArgs axis_drift_args(const qx_ctx *c)
     m.ftab = c->ftab;
     m.inv_s = 1.0 / (double)c->N;
     m.member_div = 1;
-    m.tcols = 1;
     return m;
 }
 MulArgs poisson_args(const qx_ctx *c)
@@ -330,27 +321,29 @@ void c2c_passes(qx_ctx *c, void *data, int sign)
     }
 }
 // the two (3D) or one (2D) leading-axis passes of psi around the drift
+// D = D_2 D_1 D_0, one pass per axis (see the file header); psi in rows on
+// entry and exit
 qx_status drift_core(qx_ctx *c, double dt)
 {
-    MulArgs none{};
     const int N = c->N;
-    MulArgs m;
-    TRY(drift_args(c, dt, &m));
+    TRY(load_phase_table(c, dt));
+    const MulArgs am = axis_drift_args(c);
+    const long long rpm = c->dim == 3 ? N : c->rows;
+    c->eng->c2r_rows(nullptr, nullptr, c->psi, c->rows, rpm, 0.0, nullptr, C2R_DRIFT, nullptr, c->stream, nullptr,
+                     c->ptab, c->ftab, 0, 1.0 / N);
     if (tiled(c)) {
-        // axis 1 into psi's column tiles, axis 0 on the tiles, axis 1 back
         const long long TC = c->eng->tile_cols();
         TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
-        MulArgs mk;
-        TRY(drift_args(c, dt, &mk, true));
-        c->eng->strided_tiles(c->psi, c->psit, N, N, -1, c->stream);
-        c->eng->strided(c->psit, N / TC, N * TC, 0, mk, c->stream);
-        c->eng->strided_tiles(c->psi, c->psit, N, N, +1, c->stream);
+        c->eng->strided_tiles_drift(c->psi, c->psit, N, N, am, c->stream);
+        c->eng->strided(c->psit, N / TC, N * TC, 0, am, c->stream);
+        // back to rows: a plain copy through the tile reader
+        c->eng->c2r_rows(nullptr, nullptr, c->psi, c->rows, rpm, 0.0, nullptr, C2R_SRC_TILES, nullptr, c->stream,
+                         c->psit);
     } else if (c->dim == 3) {
-        c->eng->strided(c->psi, N, N, -1, none, c->stream);
-        c->eng->strided(c->psi, 1, (long long)N * N, 0, m, c->stream);
-        c->eng->strided(c->psi, N, N, +1, none, c->stream);
+        c->eng->strided(c->psi, N, N, 0, am, c->stream);
+        c->eng->strided(c->psi, 1, (long long)N * N, 0, am, c->stream);
     } else {
-        c->eng->strided(c->psi, 1, N, 0, m, c->stream);
+        c->eng->strided(c->psi, 1, N, 0, am, c->stream);
     }
     return QX_OK;
 }
@@ -457,7 +450,7 @@ int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 7 : 4; }
 
 // pass p of one fused KDK step on psi (row layout), its column tiles psit,
 // vt (plane-major) and vt's k-major scratch vtk; see the file header.  am
-// is axis_drift_args(c), the drift table prepared for ds (drift_args)
+// is axis_drift_args(c), the drift table prepared for ds (load_phase_table)
 void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, void *vtk, const MulArgs &am,
                const MulArgs &pm)
 {
@@ -692,10 +685,7 @@ qx_status qx_kick(qx_ctx *c, double dt)
 qx_status qx_drift(qx_ctx *c, double dt)
 {
     CHECK_CTX(c);
-    MulArgs none{};
-    c->eng->contig(c->psi, c->rows, -1, none, c->stream);
     TRY(drift_core(c, dt));
-    c->eng->contig(c->psi, c->rows, +1, none, c->stream);
     return sync(c);
 }
 
@@ -867,8 +857,7 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
         return QX_OK;
     }
     const MulArgs pm = poisson_args(c);
-    MulArgs dm;
-    TRY(drift_args(c, ds, &dm));   // the phase table for ds
+    TRY(load_phase_table(c, ds));
     const MulArgs am = axis_drift_args(c);
     const bool t = tiled(c);
     if (t) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
@@ -972,8 +961,7 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     HIPCHK(hipMemcpyAsync(c->aux_c, c->psi, (size_t)c->S * c->csz, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->aux_vt, c->vt, (size_t)c->H * c->csz, hipMemcpyDeviceToDevice, c->stream));
     const MulArgs pm = poisson_args(c);
-    MulArgs dm;
-    TRY(drift_args(c, ds, &dm));
+    TRY(load_phase_table(c, ds));
     const MulArgs am = axis_drift_args(c);
     if (tiled(c)) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
@@ -1280,7 +1268,6 @@ MulArgs ens_drift_args(const EnsClass &k)
     m.ftab = ens_ftab(k);
     m.ptab_stride = k.N;
     m.member_div = 1;
-    m.tcols = 1;
     return m;
 }
 MulArgs ens_poisson_args(const qx_ens *e, const EnsClass &k)
