@@ -48,14 +48,17 @@ template <int N, typename R> struct Geom {
     // staging rows)
     static constexpr int PAIR_SLOTS = P::PADDED > N + 2 ? P::PADDED : N + 2;
     static constexpr int NP2 = (81920 - N * (int)sizeof(C)) / (PAIR_SLOTS * (int)sizeof(C));
-    static constexpr int NP = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
+    // a power of two, so row blocks tile the planes and their k-major
+    // segments stay aligned
+    static constexpr int NP3 = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
+    static constexpr int NP = 1 << ilog2(NP3);
     static constexpr int TL = 2 * NP;                 // rows per block
     static constexpr int ROW_THREADS = NP * P::TPL;
     // c2r_rows transforms a pair's two psi rows in lockstep, each with its
     // own exchange line, where two lines per pair fit in LDS beside the
     // twiddles (else one after the other through one line)
     static constexpr bool TWO_LINES =
-        (long long)(2 * NP * P::PADDED + N) * (long long)sizeof(C) + 256 <= 163840;
+        (long long)(2 * NP * P::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
     static constexpr int XCH1 = NP * P::PADDED;                        // one line per pair
     static constexpr int XCH = (TWO_LINES ? 2 : 1) * XCH1;             // c2r_rows' exchange slots
     // staging of the half spectrum, [k][row] at pitch TL + 1 (TL >= 8): a
