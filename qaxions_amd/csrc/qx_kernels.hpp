@@ -156,15 +156,19 @@ __device__ __forceinline__ void sincosm1_kick(float x, float *s, float *cm1)
         *cm1 = x2 * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
                                    4.1666667e-2f), -0.5f);
     } else {
-        float c, sh = sinf(0.5f * x);
-        sincosf(x, s, &c);
+        float sh, ch;
+        sincosf(0.5f * x, &sh, &ch);
+        *s = 2.0f * sh * ch;
         *cm1 = -2.0f * sh * sh;
     }
 }
+// FP64: one sincos of x/2 gives both, sin x = 2 sin(x/2) cos(x/2) and
+// cos x - 1 = -2 sin^2(x/2)
 __device__ __forceinline__ void sincosm1_kick(double x, double *s, double *cm1)
 {
-    double c, sh = sin(0.5 * x);
-    sincos(x, s, &c);
+    double sh, ch;
+    sincos(0.5 * x, &sh, &ch);
+    *s = 2.0 * sh * ch;
     *cm1 = -2.0 * sh * sh;
 }
 // psi * e^{ix} = psi + psi (e^{ix} - 1)
