@@ -162,14 +162,38 @@ __device__ __forceinline__ void sincosm1_kick(float x, float *s, float *cm1)
         *cm1 = -2.0f * sh * sh;
     }
 }
-// FP64: one sincos of x/2 gives both, sin x = 2 sin(x/2) cos(x/2) and
-// cos x - 1 = -2 sin^2(x/2)
+// FP64: |x| <= pi/4, Taylor polynomials to x^17 / x^18 (truncation below
+// 1e-19 absolute); otherwise one sincos of x/2 gives both, sin x =
+// 2 sin(x/2) cos(x/2) and cos x - 1 = -2 sin^2(x/2)
 __device__ __forceinline__ void sincosm1_kick(double x, double *s, double *cm1)
 {
-    double sh, ch;
-    sincos(0.5 * x, &sh, &ch);
-    *s = 2.0 * sh * ch;
-    *cm1 = -2.0 * sh * sh;
+    if (fabs(x) <= 0.78539816339744831) {
+        const double y = x * x;
+        double p = -1.0 / 355687428096000.0;           // -1/17!
+        p = fma(p, y, 1.0 / 1307674368000.0);          //  1/15!
+        p = fma(p, y, -1.0 / 6227020800.0);            // -1/13!
+        p = fma(p, y, 1.0 / 39916800.0);               //  1/11!
+        p = fma(p, y, -1.0 / 362880.0);                // -1/9!
+        p = fma(p, y, 1.0 / 5040.0);                   //  1/7!
+        p = fma(p, y, -1.0 / 120.0);                   // -1/5!
+        p = fma(p, y, 1.0 / 6.0);                      //  1/3!
+        *s = fma(-x * y, p, x);                        // x - x y (1/3! - y/5! + ...)
+        double q = -1.0 / 6402373705728000.0;          // -1/18!
+        q = fma(q, y, 1.0 / 20922789888000.0);         //  1/16!
+        q = fma(q, y, -1.0 / 87178291200.0);           // -1/14!
+        q = fma(q, y, 1.0 / 479001600.0);              //  1/12!
+        q = fma(q, y, -1.0 / 3628800.0);               // -1/10!
+        q = fma(q, y, 1.0 / 40320.0);                  //  1/8!
+        q = fma(q, y, -1.0 / 720.0);                   // -1/6!
+        q = fma(q, y, 1.0 / 24.0);                     //  1/4!
+        q = fma(q, y, -0.5);                           // -1/2!
+        *cm1 = y * q;
+    } else {
+        double sh, ch;
+        sincos(0.5 * x, &sh, &ch);
+        *s = 2.0 * sh * ch;
+        *cm1 = -2.0 * sh * sh;
+    }
 }
 // psi * e^{ix} = psi + psi (e^{ix} - 1)
 template <typename C, typename R>
