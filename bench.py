@@ -159,7 +159,8 @@ def main():
     step_achieved = B_site * S / (ms_step * 1e-3)
 
     out = {
-        "metric": "site-updates/s per KDK step at 512^3 (FP32 state, FP64 phase reduction and sums)",
+        "metric": (f"site-updates/s per KDK step at {N}^{dim} (FP32 state, FP64 phase reduction and sums)"
+                   if args.precision == "fp32" else f"site-updates/s per KDK step at {N}^{dim} (FP64)"),
         "value": value,
         "unit": "site-updates/s",
         "n_gpus": 1,
