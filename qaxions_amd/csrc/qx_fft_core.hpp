@@ -104,8 +104,10 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
-template <int N> struct Plan {
-    static constexpr int E = N < 8 ? N : 8;      // values per thread
+// EV values per thread: 8 by default (16 for the long contiguous lines of
+// the row kernels, so that a line of 1024 points is one wavefront)
+template <int N, int EV = (N < 8 ? N : 8)> struct Plan {
+    static constexpr int E = EV;                 // values per thread
     static constexpr int TPL = N / E;            // threads per line
     static constexpr int LG = ilog2(N);
     static constexpr int NST = (LG + 2) / 3;     // number of stages
@@ -131,9 +133,9 @@ struct LdsCols {    // [pos][column], columns interleaved
 // own LDS operations (and keeping the compiler from moving memory accesses
 // across the wait) is enough.  Column-interleaved lines span wavefronts and
 // take a block barrier.
-template <int N, typename L> __device__ __forceinline__ void line_sync()
+template <int N, typename L, int EV> __device__ __forceinline__ void line_sync()
 {
-    if constexpr (std::is_same<L, LdsContig>::value && Plan<N>::TPL <= 64)
+    if constexpr (std::is_same<L, LdsContig>::value && Plan<N, EV>::TPL <= 64)
         __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
     else
         __syncthreads();
@@ -141,10 +143,10 @@ template <int N, typename L> __device__ __forceinline__ void line_sync()
 
 // One Stockham stage.  v holds x[t + q TPL]; on exit (not last) v holds the
 // next stage's inputs, read back from LDS; (last) v holds y[t + q TPL].
-template <int N, int S, int DIR, typename L, typename C>
-__device__ __forceinline__ void fft_stage(C (&v)[Plan<N>::E], int t, C *line, int tc, const C *tw)
+template <int N, int S, int DIR, typename L, typename C, int EV>
+__device__ __forceinline__ void fft_stage(C (&v)[EV], int t, C *line, int tc, const C *tw)
 {
-    using P = Plan<N>;
+    using P = Plan<N, EV>;
     constexpr int E = P::E, TPL = P::TPL;
     constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
     constexpr bool last = (S == P::NST - 1);
@@ -179,17 +181,17 @@ __device__ __forceinline__ void fft_stage(C (&v)[Plan<N>::E], int t, C *line, in
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = out[q];
     } else {
-        line_sync<N, L>();
+        line_sync<N, L, EV>();
 #pragma unroll
         for (int q = 0; q < E; ++q) v[q] = line[L::at(t + q * TPL, tc)];
-        line_sync<N, L>();
+        line_sync<N, L, EV>();
     }
 }
 
-template <int N, int S, int DIR, typename L, typename C>
-__device__ __forceinline__ void stage_loop(C (&v)[Plan<N>::E], int t, C *line, int tc, const C *tw)
+template <int N, int S, int DIR, typename L, typename C, int EV>
+__device__ __forceinline__ void stage_loop(C (&v)[EV], int t, C *line, int tc, const C *tw)
 {
-    if constexpr (S < Plan<N>::NST) {
+    if constexpr (S < Plan<N, EV>::NST) {
         fft_stage<N, S, DIR, L>(v, t, line, tc, tw);
         stage_loop<N, S + 1, DIR, L>(v, t, line, tc, tw);
     }
@@ -200,8 +202,8 @@ __device__ __forceinline__ void stage_loop(C (&v)[Plan<N>::E], int t, C *line, i
 // with block barriers).  `line` is this line's LDS
 // area, `tc` the column interleave for LdsCols, `tw` the LDS twiddle table
 // tw[m] = e^{-2 pi i m/N}.
-template <int N, int DIR, typename L, typename C>
-__device__ __forceinline__ void fft_line(C (&v)[Plan<N>::E], int t, C *line, int tc, const C *tw)
+template <int N, int DIR, typename L, typename C, int EV>
+__device__ __forceinline__ void fft_line(C (&v)[EV], int t, C *line, int tc, const C *tw)
 {
     stage_loop<N, 0, DIR, L>(v, t, line, tc, tw);
 }
@@ -210,11 +212,11 @@ __device__ __forceinline__ void fft_line(C (&v)[Plan<N>::E], int t, C *line, int
 // butterflies of both lines, one synchronisation per exchange for both, two
 // independent dependency chains for the scheduler.  line_a / line_b are the
 // two lines' LDS areas.
-template <int N, int S, int DIR, typename L, typename C>
-__device__ __forceinline__ void fft_stage2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>::E], int t, C *line_a,
+template <int N, int S, int DIR, typename L, typename C, int EV>
+__device__ __forceinline__ void fft_stage2(C (&va)[EV], C (&vb)[EV], int t, C *line_a,
                                            C *line_b, int tc, const C *tw)
 {
-    using P = Plan<N>;
+    using P = Plan<N, EV>;
     constexpr int E = P::E, TPL = P::TPL;
     constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
     constexpr bool last = (S == P::NST - 1);
@@ -263,28 +265,28 @@ __device__ __forceinline__ void fft_stage2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>:
             vb[q] = ob[q];
         }
     } else {
-        line_sync<N, L>();
+        line_sync<N, L, EV>();
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             va[q] = line_a[L::at(t + q * TPL, tc)];
             vb[q] = line_b[L::at(t + q * TPL, tc)];
         }
-        line_sync<N, L>();
+        line_sync<N, L, EV>();
     }
 }
 
-template <int N, int S, int DIR, typename L, typename C>
-__device__ __forceinline__ void stage_loop2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>::E], int t, C *line_a,
+template <int N, int S, int DIR, typename L, typename C, int EV>
+__device__ __forceinline__ void stage_loop2(C (&va)[EV], C (&vb)[EV], int t, C *line_a,
                                             C *line_b, int tc, const C *tw)
 {
-    if constexpr (S < Plan<N>::NST) {
+    if constexpr (S < Plan<N, EV>::NST) {
         fft_stage2<N, S, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
         stage_loop2<N, S + 1, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
     }
 }
 
-template <int N, int DIR, typename L, typename C>
-__device__ __forceinline__ void fft_line2(C (&va)[Plan<N>::E], C (&vb)[Plan<N>::E], int t, C *line_a,
+template <int N, int DIR, typename L, typename C, int EV>
+__device__ __forceinline__ void fft_line2(C (&va)[EV], C (&vb)[EV], int t, C *line_a,
                                           C *line_b, int tc, const C *tw)
 {
     stage_loop2<N, 0, DIR, L>(va, vb, t, line_a, line_b, tc, tw);

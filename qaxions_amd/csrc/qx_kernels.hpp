@@ -30,11 +30,15 @@ namespace qx {
 // launch geometry shared by host and device
 // ---------------------------------------------------------------------------
 template <int N, typename R> struct Geom {
-    using P = Plan<N>;
+    using P = Plan<N>;                  // column passes
+    // contiguous lines (contig_fft, row kernels); the same plan (16 values per
+    // thread for N >= 1024 keeps a line in one wavefront, but the row
+    // kernels' registers then cost more than the block barriers it saves)
+    using PR = Plan<N>;
     using C = typename cplx<R>::T;
     // contig_fft: lines per block
-    static constexpr int LPB = (256 / P::TPL) > 0 ? (256 / P::TPL) : 1;
-    static constexpr int CONTIG_THREADS = LPB * P::TPL;
+    static constexpr int LPB = (256 / PR::TPL) > 0 ? (256 / PR::TPL) : 1;
+    static constexpr int CONTIG_THREADS = LPB * PR::TPL;
     // strided_fft: columns per block (LDS budget 64 KiB, <= 1024 threads)
     static constexpr int TC0 = (65536 / (N * (int)sizeof(C))) < 1 ? 1 : (65536 / (N * (int)sizeof(C)));
     static constexpr int TC1 = TC0 > 16 ? 16 : TC0;
@@ -42,24 +46,24 @@ template <int N, typename R> struct Geom {
     static constexpr int STRIDED_THREADS = TC * P::TPL;
     static constexpr int TC_HALF = TC > 1 ? TC / 2 : 1;   // narrow-tile variant
     // row kernels: pairs of rows per block
-    static constexpr int NP0 = (256 / P::TPL) > 8 ? (256 / P::TPL) : 8;
-    static constexpr int NP1 = (NP0 * P::TPL > 1024) ? (1024 / P::TPL) : NP0;
+    static constexpr int NP0 = (256 / PR::TPL) > 8 ? (256 / PR::TPL) : 8;
+    static constexpr int NP1 = (NP0 * PR::TPL > 1024) ? (1024 / PR::TPL) : NP0;
     // LDS budget of ~80 KiB: per pair max(exchange line, two half-spectrum
     // staging rows)
-    static constexpr int PAIR_SLOTS = P::PADDED > N + 2 ? P::PADDED : N + 2;
+    static constexpr int PAIR_SLOTS = PR::PADDED > N + 2 ? PR::PADDED : N + 2;
     static constexpr int NP2 = (81920 - N * (int)sizeof(C)) / (PAIR_SLOTS * (int)sizeof(C));
     // a power of two, so row blocks tile the planes and their k-major
     // segments stay aligned
     static constexpr int NP3 = NP2 < 1 ? 1 : (NP2 < NP1 ? NP2 : NP1);
     static constexpr int NP = 1 << ilog2(NP3);
     static constexpr int TL = 2 * NP;                 // rows per block
-    static constexpr int ROW_THREADS = NP * P::TPL;
+    static constexpr int ROW_THREADS = NP * PR::TPL;
     // c2r_rows transforms a pair's two psi rows in lockstep, each with its
     // own exchange line, where two lines per pair fit in LDS beside the
     // twiddles (else one after the other through one line)
     static constexpr bool TWO_LINES =
-        (long long)(2 * NP * P::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
-    static constexpr int XCH1 = NP * P::PADDED;                        // one line per pair
+        (long long)(2 * NP * PR::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
+    static constexpr int XCH1 = NP * PR::PADDED;                        // one line per pair
     static constexpr int XCH = (TWO_LINES ? 2 : 1) * XCH1;             // c2r_rows' exchange slots
     // staging of the half spectrum, [k][row] at pitch TL + 1 (TL >= 8): a
     // wave reading one row at 64 consecutive k then spreads over all banks
@@ -265,7 +269,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
            long long rm_m, long long rm_a)
 {
     using C = typename cplx<R>::T;
-    using P = Plan<N>;
+    using P = typename Geom<N, R>::PR;
     using G = Geom<N, R>;
     __shared__ C lds[G::LPB * P::PADDED];
     __shared__ C tw[N];
@@ -425,7 +429,7 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
          const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
-    using P = Plan<N>;
+    using P = typename Geom<N, R>::PR;
     using G = Geom<N, R>;
     constexpr int TL = G::TL, H1 = N / 2 + 1;
     __shared__ C lds[G::ROW_LDS1];
@@ -526,7 +530,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          const float2 *__restrict__ ftab, long long ptab_stride, double inv_n)
 {
     using C = typename cplx<R>::T;
-    using P = Plan<N>;
+    using P = typename Geom<N, R>::PR;
     using G = Geom<N, R>;
     constexpr int TL = G::TL, H1 = N / 2 + 1;
     __shared__ C lds[G::ROW_LDS];
