@@ -132,6 +132,23 @@ __device__ __forceinline__ long long tile_index(long long row, int i2)
     return ((long long)(i2 / TT) * N + i0) * (long long)(N * TT) + i1 * TT + (i2 % TT);
 }
 
+// A thread's values of one row in the tiles, positions t + q*TPL: when TPL is
+// a multiple of TT they are at tile_index(row, t) + q * TILE_Q, a compile-time
+// stride, so the 64-bit index is formed once per row
+template <typename C, int N, int TT, int TPL, int E>
+__device__ __forceinline__ void load_tile_row(C (&v)[E], const C *__restrict__ tiles, long long row, int t, bool ok)
+{
+    if constexpr (TPL % TT == 0) {
+        const C *p = tiles + tile_index<N, TT>(row, t);
+        constexpr long long QS = (long long)(TPL / TT) * N * (long long)(N * TT);
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = ok ? p[q * QS] : C{0, 0};
+    } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = ok ? tiles[tile_index<N, TT>(row, t + q * TPL)] : C{0, 0};
+    }
+}
+
 // index of (row, k_last) in vt for members of `rpm` rows stored one after
 // another: member-major, then k-major inside a member
 __device__ __forceinline__ long long vt_index(long long row, int k, long long rpm, int h1)
@@ -451,13 +468,13 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
         __syncthreads();
     } else {
         C a[P::E], b[P::E];
+        if constexpr (MODE == R2C_DENSITY_TILES) {
+            load_tile_row<C, N, G::TC, P::TPL>(a, psi, ra, t, oka);
+            load_tile_row<C, N, G::TC, P::TPL>(b, psi, rb, t, okb);
+        } else {
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) {
-            const int pos = t + q * P::TPL;
-            if constexpr (MODE == R2C_DENSITY_TILES) {
-                a[q] = oka ? psi[tile_index<N, G::TC>(ra, pos)] : C{0, 0};
-                b[q] = okb ? psi[tile_index<N, G::TC>(rb, pos)] : C{0, 0};
-            } else {
+            for (int q = 0; q < P::E; ++q) {
+                const int pos = t + q * P::TPL;
                 a[q] = oka ? psi[ra * N + pos] : C{0, 0};
                 b[q] = okb ? psi[rb * N + pos] : C{0, 0};
             }
@@ -547,12 +564,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     C a[P::E], b[P::E];
     if (upd) {
         if (flags & C2R_SRC_TILES) {
-#pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                const int pos = t + q * P::TPL;
-                a[q] = oka ? tiles[tile_index<N, G::TC>(ra, pos)] : C{0, 0};
-                b[q] = okb ? tiles[tile_index<N, G::TC>(rb, pos)] : C{0, 0};
-            }
+            load_tile_row<C, N, G::TC, P::TPL>(a, tiles, ra, t, oka);
+            load_tile_row<C, N, G::TC, P::TPL>(b, tiles, rb, t, okb);
         } else {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
