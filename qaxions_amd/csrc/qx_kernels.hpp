@@ -4,23 +4,28 @@
 //   psi  : S complex, row-major [i0][i1][i2], last index fastest (as the
 //          reference's fftw_complex array, src/field.h:92);
 //   V    : S real, same order (src/field.h:94);
-//   vt   : the half spectrum of the real potential, stored k-major:
-//          vt[k_last * rows + row] for k_last in [0, N/2] and row the
-//          flattened leading indices.  Putting k_last outermost makes every
-//          remaining transform a contiguous or N-strided pass and turns the
-//          real<->complex passes into row-tile transposes (the reference's
-//          own layout, [row][k_last], is produced only at the API boundary).
+//   vt   : the half spectrum of the real potential, plane-major: member (2D
+//          ensembles) or i0 plane (3D) outermost, then k_last, then the
+//          remaining row index (vt_index); a row block of the real<->complex
+//          passes then reads and writes one plane, and every other transform
+//          of the Poisson solve is a contiguous or N-strided pass.  The
+//          reference's own layout, [row][k_last], exists only at the API
+//          boundary;
+//   psit : (3D, inside a step) psi in column tiles [i2/TC][i0][i1][i2%TC]
+//          (tile_index), which makes the i0 axis a column pass of stride N*TC.
 //
 // Pass kinds:
-//   contig_fft   : lines along the contiguous axis, optional fused multiply
+//   contig_fft   : lines along the contiguous axis, optionally out of place
+//                  with swapped outer indices, optional fused multiply
 //                  (forward, pointwise factor, inverse) for the 2D potential;
-//   strided_fft  : columns at stride `inner` (the leading axes), optional
-//                  fused drift or Poisson multiply;
-//   r2c_rows     : real rows -> vt, fused with the inverse transform of psi
-//                  and the density pref(|psi|^2 - rho_bar) (two real rows
-//                  packed in one complex transform);
+//   strided_fft  : columns at stride `inner`, optional fused one-axis drift
+//                  or Poisson multiply, optionally writing column tiles;
+//   r2c_rows     : density pref(|psi|^2 - rho_bar) of psi rows (or of psi's
+//                  column tiles) -> vt, two real rows packed in one complex
+//                  transform;
 //   c2r_rows     : vt -> real rows, fused with max|V|, the kick
-//                  psi <- psi e^{-i dt V} and the forward transform of psi.
+//                  psi <- psi e^{-i dt V} and the drift along the last axis
+//                  (forward transform, factor, inverse).
 #pragma once
 #include "qx_fft_core.hpp"
 
