@@ -241,6 +241,7 @@ def bench_ensemble(qx, args):
     ens = qx.Ensemble(Ns, Ls, precision="fp32", pref=1.0, graph_steps=16)
     ens.ic_grf(list(range(len(Ns))))
     ens.update_potential()
+    variants = ens.tune(ds, reps=3) if not args.no_tune else None
     m0 = ens.mass()
     ens.leapfrog(ds, max(args.warmup, 1))
     ev = HipEvents()
@@ -258,7 +259,8 @@ def bench_ensemble(qx, args):
            "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "c64",
            "data": "synthetic: per-member GRF ICs, seeds 0-95",
            "config": {"workload": "32x128^2 + 32x512^2 + 24x1024^2 + 8x2048^2, L~U[5,40], "
-                                  "dt=1e-3 (dx/dx0)^2, hipGraph 16 steps/replay"},
+                                  "dt=1e-3 (dx/dx0)^2, hipGraph 16 steps/replay",
+                      "tile_variants": {str(k): v for k, v in variants.items()} if variants else None},
            "step_roofline": {"bound": "hbm", "algorithmic_bytes": B,
                              "achieved": B / (ms / steps * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                              "unit": "GB/s", "frac": B / (ms / steps * 1e-3) / HBM_PEAK},

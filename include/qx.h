@@ -171,6 +171,13 @@ qx_status qx_ens_ic_grf(qx_ens *ens, const uint64_t *seeds, double amp);
 qx_status qx_ens_update_potential(qx_ens *ens);
 /* qx_leapfrog for every member at once; ds[member]; nsteps >= 1 */
 qx_status qx_ens_leapfrog(qx_ens *ens, const double *ds, int nsteps, int first_half, int last_half);
+/* Autotuner for ensembles: per size class, the faster of the two tile
+ * variants of its strided drift pass, timed over `reps` steady steps of the
+ * class on saved copies of its state (the state is restored).  variants
+ * (nullable): one int per class, classes in increasing N */
+qx_status qx_ens_tune(qx_ens *ens, const double *ds, int reps, int *variants);
+int qx_ens_classes(const qx_ens *ens);
+int qx_ens_class_size(const qx_ens *ens, int cls);
 /* mass[member] = (L/N)^2 sum |psi|^2 */
 qx_status qx_ens_mass(qx_ens *ens, double *mass);
 int qx_ens_size(const qx_ens *ens);

@@ -91,6 +91,9 @@ SIGNATURES = {
     "qx_ens_ic_grf": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_double]),
     "qx_ens_update_potential": (C.c_int, [C.c_void_p]),
     "qx_ens_leapfrog": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int]),
+    "qx_ens_tune": (C.c_int, [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "qx_ens_classes": (C.c_int, [C.c_void_p]),
+    "qx_ens_class_size": (C.c_int, [C.c_void_p, C.c_int]),
     "qx_ens_mass": (C.c_int, [C.c_void_p, _dp]),
     "qx_ens_size": (C.c_int, [C.c_void_p]),
     "qx_ens_sites": (C.c_int64, [C.c_void_p]),
@@ -457,6 +460,15 @@ class Ensemble:
 
     def update_potential(self):
         _check(lib().qx_ens_update_potential(self._h))
+
+    def tune(self, ds, reps=3):
+        """Autotune each size class's drift tile by timing its steps (the
+        state is kept); returns {N: variant} (0 full tile, 1 half tile)"""
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(ds, np.float64), (len(self.Ns),)))
+        n = int(lib().qx_ens_classes(self._h))
+        v = (C.c_int * max(n, 1))()
+        _check(lib().qx_ens_tune(self._h, _p(d), int(reps), v))
+        return {int(lib().qx_ens_class_size(self._h, i)): int(v[i]) for i in range(n)}
 
     def leapfrog(self, ds, nsteps, first_half=True, last_half=True):
         n = len(self.Ns)

@@ -1553,6 +1553,63 @@ qx_status qx_ens_update_potential(qx_ens *e)
     return ens_sync(e);
 }
 
+// per size class, the faster tile variant of its strided drift pass, timed
+// over `reps` steady steps of the class on saved state (restored afterwards)
+qx_status qx_ens_tune(qx_ens *e, const double *ds, int reps, int *variants)
+{
+    if (!e || !ds) return fail(QX_EINVAL, "null argument");
+    if (reps < 1) return fail(QX_EINVAL, "reps must be >= 1");
+    HIPCHK(hipSetDevice(e->device));
+    TRY(ens_set_ds(e, ds));
+    ens_free_graph(e);   // captured launches would keep the old tiles
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    qx_status st = QX_OK;
+    for (size_t ci = 0; ci < e->classes.size() && st == QX_OK; ++ci) {
+        EnsClass &k = e->classes[ci];
+        const size_t bp = (size_t)(k.B * k.S) * e->csz, bv = (size_t)(k.B * k.H) * e->csz;
+        void *sp = nullptr, *sv = nullptr;
+        if (hipMalloc(&sp, bp) != hipSuccess || hipMalloc(&sv, bv) != hipSuccess) {
+            st = fail(QX_ENOMEM, "tuning copies");
+        } else if (hipMemcpyAsync(sp, k.psi, bp, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
+                   hipMemcpyAsync(sv, k.vt, bv, hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
+            st = fail(QX_EDEVICE, "tuning copies");
+        }
+        float ms[2] = {0.f, 0.f};
+        for (int v = 0; v < 2 && st == QX_OK; ++v) {
+            for (auto &row : k.eng->strided_variant) row[0] = row[1] = v;
+            ens_class_step(e, k, e->stream);   // warm
+            if (hipEventRecord(e0, e->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
+            for (int r = 0; r < reps; ++r) ens_class_step(e, k, e->stream);
+            if (hipEventRecord(e1, e->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
+            if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess))
+                st = fail(QX_EDEVICE, "event timing failed");
+        }
+        const int best = ms[1] < ms[0] ? 1 : 0;
+        for (auto &row : k.eng->strided_variant) row[0] = row[1] = best;
+        if (variants) variants[ci] = best;
+        if (sp && sv && st == QX_OK) {
+            if (hipMemcpyAsync(k.psi, sp, bp, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
+                hipMemcpyAsync(k.vt, sv, bv, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
+                hipStreamSynchronize(e->stream) != hipSuccess)
+                st = fail(QX_EDEVICE, "restoring state after tuning");
+        }
+        (void)hipStreamSynchronize(e->stream);
+        if (sp) (void)hipFree(sp);
+        if (sv) (void)hipFree(sv);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+int qx_ens_classes(const qx_ens *e) { return e ? (int)e->classes.size() : 0; }
+int qx_ens_class_size(const qx_ens *e, int ci)
+{
+    return (e && ci >= 0 && ci < (int)e->classes.size()) ? e->classes[ci].N : 0;
+}
+
 qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_half, int last_half)
 {
     if (!e || !ds) return fail(QX_EINVAL, "null argument");

@@ -367,3 +367,27 @@ def test_ensemble_members_match_solo_runs(qx, prec, graph_steps):
     V0, _, _ = o.update_potential(psi0, Ls[1], 1.0)
     want, _ = o.leapfrog(psi0, V0, Ls[1], ds[1], 9, 1.0)
     assert rel_l2(ens.psi(1), want) < (1e-12 if prec == "fp64" else 5e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_ensemble_tune_keeps_state_and_results(qx, prec):
+    """The ensemble autotuner reports one tile variant per size class, leaves
+    every member's state as it was, and the tile variant does not change
+    results (each column's transform is the same whatever the tile width)."""
+    Ns = [16, 64, 16, 32]
+    Ls = [5.0, 12.0, 40.0, 7.5]
+    ds = [1e-3 * (L / N / (10.0 / 256)) ** 2 for N, L in zip(Ns, Ls)]
+    a = qx.Ensemble(Ns, Ls, precision=prec, pref=1.0, graph_steps=0)
+    b = qx.Ensemble(Ns, Ls, precision=prec, pref=1.0, graph_steps=0)
+    for e in (a, b):
+        e.ic_grf([21, 22, 23, 24])
+        e.update_potential()
+    before = [a.psi(i) for i in range(len(Ns))]
+    v = a.tune(ds, reps=2)
+    assert sorted(v) == sorted(set(Ns)) and set(v.values()) <= {0, 1}
+    for i in range(len(Ns)):
+        assert np.array_equal(a.psi(i), before[i])
+    a.leapfrog(ds, 5)
+    b.leapfrog(ds, 5)
+    for i in range(len(Ns)):
+        assert np.array_equal(a.psi(i), b.psi(i))
