@@ -107,6 +107,17 @@ qx_status qx_spectrum(qx_ctx *ctx, int source, int nbins, int spb, double *Pk, i
  * open), then nsteps x [D(ds), P] with K(ds) between steps, and closes with
  * K(ds/2) if last_half.  Leaves V current.  Runs the fused pass schedule. */
 qx_status qx_leapfrog(qx_ctx *ctx, double ds, int nsteps, int first_half, int last_half);
+/* The same loop with constants that change from step to step: the MRE run
+ * of the reference, whose Field::updateTime (src/field.cxx:349-361) sets a
+ * new ds and a after every drift, so that pref = norm_ a (:298) changes
+ * too.  Requires V current.  K(kick_dt[0]); then for step j < nsteps:
+ * D(drift_dt[j]), P with pref[j] (the context's rho_bar and k = 0 mode),
+ * K(kick_dt[j+1]); kick_dt[nsteps] = 0 leaves the segment open.  Arrays:
+ * drift_dt[nsteps], kick_dt[nsteps+1], pref[nsteps].  The context's
+ * physics are not changed.  Fused pass schedule, one step's tables per
+ * launch set (not graph-captured). */
+qx_status qx_leapfrog_schedule(qx_ctx *ctx, int nsteps, const double *drift_dt, const double *kick_dt,
+                               const double *pref);
 /* Field::propagate (src/field.cxx:388-402): K(ds/2) D(ds) P K(ds/2) */
 qx_status qx_propagate(qx_ctx *ctx, double ds);
 

@@ -59,6 +59,9 @@ def lib():
             "qxo_spectrum": (None, [i, i, d, _dp, i, i, _dp, _i64p, _dp]),
             "qxo_leapfrog": (None, [i, i, d, C.POINTER(Physics), _dp, _dp, _dp, d, i, i, i]),
             "qxo_propagate": (None, [i, i, d, C.POINTER(Physics), _dp, _dp, _dp, d]),
+            "qxo_mre_schedule": (None, [d, d, i, i, i, _dp, _dp, _dp, _dp, _dp]),
+            "qxo_leapfrog_schedule": (None, [i, i, d, C.POINTER(Physics), _dp, _dp, _dp, i,
+                                             _dp, _dp, _dp]),
             "qxo_meas_list": (i, [C.c_int64, C.c_int64, _i64p]),
             "qxo_ic_solitons": (None, [i, i, d, d, _dp, i, _dp]),
             "qxo_rng_u64": (u64, [u64, u64, u64]),
@@ -223,6 +226,32 @@ def propagate(psi, V, L, ds, pref, rho_bar=1.0, k0_mode=1, rho_bar_mode=0):
     ph = Physics(pref, rho_bar, k0_mode, rho_bar_mode)
     lib().qxo_propagate(dim, N, float(L), C.byref(ph), _p(psi.view(np.float64)), _p(V),
                         _p(Vh.view(np.float64)), float(ds))
+    return psi, V
+
+
+def mre_schedule(ai, dt, nsteps, first_half=1, last_half=1):
+    """per-step constants of an MRE run: (drift_dt[n], kick_dt[n+1], pref[n], a, s)"""
+    n = int(nsteps)
+    dd, kk, pp = np.zeros(n), np.zeros(n + 1), np.zeros(n)
+    a, s = C.c_double(), C.c_double()
+    lib().qxo_mre_schedule(float(ai), float(dt), n, int(first_half), int(last_half), _p(dd),
+                           _p(kk), _p(pp), C.byref(a), C.byref(s))
+    return dd, kk, pp, a.value, s.value
+
+
+def leapfrog_schedule(psi, V, L, drift_dt, kick_dt, pref, rho_bar=1.0, k0_mode=1,
+                      rho_bar_mode=0):
+    psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    V = np.ascontiguousarray(V, dtype=np.float64).copy()
+    dim, N = psi.ndim, psi.shape[0]
+    Vh = np.zeros(_half_shape(dim, N), np.complex128)
+    dd = np.ascontiguousarray(drift_dt, np.float64)
+    kk = np.ascontiguousarray(kick_dt, np.float64)
+    pp = np.ascontiguousarray(pref, np.float64)
+    assert kk.size == dd.size + 1 and pp.size == dd.size
+    ph = Physics(0.0, rho_bar, k0_mode, rho_bar_mode)
+    lib().qxo_leapfrog_schedule(dim, N, float(L), C.byref(ph), _p(psi.view(np.float64)), _p(V),
+                                _p(Vh.view(np.float64)), int(dd.size), _p(dd), _p(kk), _p(pp))
     return psi, V
 
 

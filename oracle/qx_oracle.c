@@ -427,6 +427,49 @@ void qxo_leapfrog(int dim, int N, double L, const qxo_physics *ph, double *psi, 
     if (last_half) qxo_kick(dim, N, psi, V, 0.5 * ds);
 }
 
+/* Field::setCosmo MRE branch (src/field.cxx:82-89) and Field::updateTime
+ * (src/field.cxx:349-361) unrolled over the merged-kick loop of
+ * src/main.cxx:65-85: drift_dt[j] = ds before step j's updateTime, pref[j] =
+ * norm_ a after it (norm_ = 1 in MRE, src/field.cxx:88, pref at :298),
+ * kick_dt[0] the opening kick, kick_dt[j] (0 < j < n) the merged kick
+ * between steps j-1 and j, kick_dt[n] the closing kick (0 when absent).
+ * a_out/s_out: a and s after the n steps. */
+void qxo_mre_schedule(double ai, double dt, int nsteps, int first_half, int last_half,
+                      double *drift_dt, double *kick_dt, double *pref, double *a_out,
+                      double *s_out)
+{
+    double a = ai, ds = dt / ai, s = 0.0;
+    kick_dt[0] = first_half ? 0.5 * ds : ds;
+    for (int j = 0; j < nsteps; ++j) {
+        drift_dt[j] = ds;
+        ds = 0.00025 / a;
+        a = a * (1 + 1.27 * 1.27 * sqrt(1 + a) * ds);
+        s += ds;
+        pref[j] = a;
+        kick_dt[j + 1] = (j != nsteps - 1) ? ds : (last_half ? 0.5 * ds : 0.0);
+    }
+    *a_out = a;
+    *s_out = s;
+}
+
+/* the merged-kick loop of src/main.cxx:65-85 with per-step constants (the
+ * MRE run, where Field::updateTime changes ds and a every step): requires V
+ * current; K(kick_dt[0]); then for each step j: D(drift_dt[j]), P with
+ * pref[j], and K(kick_dt[j+1]) (skipped when 0) */
+void qxo_leapfrog_schedule(int dim, int N, double L, const qxo_physics *ph, double *psi,
+                           double *V, double *Vhat, int nsteps, const double *drift_dt,
+                           const double *kick_dt, const double *pref)
+{
+    qxo_kick(dim, N, psi, V, kick_dt[0]);
+    for (int j = 0; j < nsteps; ++j) {
+        qxo_physics pj = *ph;
+        pj.pref = pref[j];
+        qxo_drift(dim, N, L, psi, drift_dt[j]);
+        update_pot(dim, N, L, &pj, psi, V, Vhat);
+        if (kick_dt[j + 1] != 0.0) qxo_kick(dim, N, psi, V, kick_dt[j + 1]);
+    }
+}
+
 /* Field::propagate (src/field.cxx:388-402) */
 void qxo_propagate(int dim, int N, double L, const qxo_physics *ph, double *psi, double *V,
                    double *Vhat, double ds)

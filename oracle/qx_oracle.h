@@ -83,6 +83,15 @@ typedef struct {
 void qxo_leapfrog(int dim, int N, double L, const qxo_physics *ph, double *psi,
                   double *V, double *Vhat, double ds, int nsteps, int first_half,
                   int last_half);
+/* MRE cosmology (src/field.cxx:82-89, 349-361) unrolled into per-step
+ * constants for qxo_leapfrog_schedule: drift_dt[n], kick_dt[n+1], pref[n] */
+void qxo_mre_schedule(double ai, double dt, int nsteps, int first_half, int last_half,
+                      double *drift_dt, double *kick_dt, double *pref, double *a_out,
+                      double *s_out);
+/* merged-kick leapfrog with per-step drift dt, potential prefactor and kick */
+void qxo_leapfrog_schedule(int dim, int N, double L, const qxo_physics *ph, double *psi,
+                           double *V, double *Vhat, int nsteps, const double *drift_dt,
+                           const double *kick_dt, const double *pref);
 /* propagate(): unmerged K/2 D P K/2 (src/field.cxx:388-402) */
 void qxo_propagate(int dim, int N, double L, const qxo_physics *ph, double *psi,
                    double *V, double *Vhat, double ds);

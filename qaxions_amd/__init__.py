@@ -69,6 +69,7 @@ SIGNATURES = {
     "qx_vmax": (C.c_double, [C.c_void_p]),
     "qx_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _i64p, _dp]),
     "qx_leapfrog": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
+    "qx_leapfrog_schedule": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "qx_propagate": (C.c_int, [C.c_void_p, C.c_double]),
     "qx_fft_c2c": (C.c_int, [C.c_void_p, _dp, C.c_int]),
     "qx_fft_r2c": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -306,6 +307,16 @@ class Lattice:
     def leapfrog(self, ds, nsteps, first_half=True, last_half=True):
         _check(lib().qx_leapfrog(self._h, float(ds), int(nsteps), int(bool(first_half)),
                                  int(bool(last_half))))
+
+    def leapfrog_schedule(self, drift_dt, kick_dt, pref):
+        """merged-kick loop with per-step constants: K(kick_dt[0]), then per
+        step j D(drift_dt[j]), P with pref[j], K(kick_dt[j+1]) (0: none)"""
+        dd = np.ascontiguousarray(drift_dt, np.float64)
+        kk = np.ascontiguousarray(kick_dt, np.float64)
+        pp = np.ascontiguousarray(pref, np.float64)
+        if kk.size != dd.size + 1 or pp.size != dd.size:
+            raise ValueError("need len(kick_dt) == len(drift_dt) + 1 == len(pref) + 1")
+        _check(lib().qx_leapfrog_schedule(self._h, int(dd.size), _p(dd), _p(kk), _p(pp)))
 
     def propagate(self, ds):
         _check(lib().qx_propagate(self._h, float(ds)))

@@ -138,6 +138,9 @@ struct qx_ctx {
                                // e^{i theta} - 1 (N float2, ftab)
     float2 *ftab = nullptr;
     double ptab_dt = std::nan("");
+    // a scheduled leapfrog's per-step drift tables, SCHED_CHUNK steps at a
+    // time: step i's N double2 then N float2 at i * N * 24 bytes (lazy)
+    unsigned char *sched = nullptr;
     // captured steady-state steps for small lattices (launch-bound)
     hipGraphExec_t gexec = nullptr;
     double g_ds = std::nan("");
@@ -449,10 +452,12 @@ bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
 int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 7 : 4; }
 
 // pass p of one fused KDK step on psi (row layout), its column tiles psit,
-// vt (plane-major) and vt's k-major scratch vtk; see the file header.  am
-// is axis_drift_args(c), the drift table prepared for ds (load_phase_table)
-void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, void *vtk, const MulArgs &am,
-               const MulArgs &pm)
+// vt (plane-major) and vt's k-major scratch vtk; see the file header.  The
+// step opens with the kick `ds` (pass 0), drifts with am's table (the
+// drift_dt table of load_phase_table, or a scheduled step's own) and forms
+// V with the prefactor `pref` (the density pass)
+void step_pass(qx_ctx *c, int p, double ds, double pref, void *psi, void *psit, void *vt, void *vtk,
+               const MulArgs &am, const MulArgs &pm)
 {
     MulArgs none{};
     const int N = c->N;
@@ -463,8 +468,8 @@ void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, voi
         const int kick = C2R_SRC_VT | C2R_KICK | C2R_DRIFT | (t ? C2R_SRC_TILES : 0);
         switch (p) {
         case 0:
-            c->eng->c2r_rows(vt, c->V, psi, c->rows, N, ds, nullptr, kick, nullptr, c->stream, psit, c->ptab,
-                             c->ftab, 0, inv_n);
+            c->eng->c2r_rows(vt, c->V, psi, c->rows, N, ds, nullptr, kick, nullptr, c->stream, psit, am.ptab,
+                             am.ftab, 0, inv_n);
             break;
         case 1:
             if (t) c->eng->strided_tiles_drift(psi, psit, N, N, am, c->stream);
@@ -475,8 +480,8 @@ void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, voi
             else c->eng->strided(psi, 1, (long long)N * N, 0, am, c->stream);
             break;
         case 3:
-            if (t) c->eng->r2c_rows(R2C_DENSITY_TILES, psit, nullptr, vt, c->rows, N, c->ph.pref, c->ph.rho_bar, c->stream);
-            else c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, N, c->ph.pref, c->ph.rho_bar, c->stream);
+            if (t) c->eng->r2c_rows(R2C_DENSITY_TILES, psit, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream);
+            else c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream);
             break;
         case 4: c->eng->contig(vt, c->H1 * N, -1, none, c->stream, vtk, c->H1, N); break;
         case 5: c->eng->strided(vtk, c->H1, N, 0, pm, c->stream); break;
@@ -487,10 +492,10 @@ void step_pass(qx_ctx *c, int p, double ds, void *psi, void *psit, void *vt, voi
         switch (p) {
         case 0:
             c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, nullptr,
-                             c->ptab, c->ftab, 0, inv_n);
+                             am.ptab, am.ftab, 0, inv_n);
             break;
         case 1: c->eng->strided(psi, 1, N, 0, am, c->stream); break;
-        case 2: c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, c->rows, c->ph.pref, c->ph.rho_bar, c->stream); break;
+        case 2: c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, c->rows, pref, c->ph.rho_bar, c->stream); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
     }
@@ -597,7 +602,7 @@ qx_status qx_destroy(qx_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->psi, c->V, c->vt, c->psit, c->vtk, c->aux_c, c->aux_vt, c->aux_r, c->stage,
-                    c->partials, c->vmax_bits, c->spec_P, c->spec_cnt, c->ptab,
+                    c->partials, c->vmax_bits, c->spec_P, c->spec_cnt, c->ptab, c->sched,
                     c->eng ? c->eng->tw : nullptr};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -831,7 +836,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &am, co
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < GRAPH_STEPS; ++i)
-                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, am, pm);
+                for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->ph.pref, c->psi, c->psit, c->vt, c->vtk, am, pm);
             HIPCHK(hipStreamEndCapture(c->stream, &g));
             const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -842,7 +847,7 @@ static qx_status steady_steps(qx_ctx *c, int n, double ds, const MulArgs &am, co
         for (; done + GRAPH_STEPS <= n; done += GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     }
     for (; done < n; ++done)
-        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, am, pm);
+        for (int p = 0; p < np; ++p) step_pass(c, p, ds, c->ph.pref, c->psi, c->psit, c->vt, c->vtk, am, pm);
     return QX_OK;
 }
 
@@ -867,7 +872,7 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
     // the opening kick reads the stored V; later kicks the potential's spectrum
     c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, rpm, k_first, nullptr, C2R_KICK | C2R_DRIFT, nullptr,
                      c->stream, nullptr, c->ptab, c->ftab, 0, 1.0 / c->N);
-    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->psi, c->psit, c->vt, c->vtk, am, pm);
+    for (int p = 1; p < np; ++p) step_pass(c, p, ds, c->ph.pref, c->psi, c->psit, c->vt, c->vtk, am, pm);
     TRY(steady_steps(c, nsteps - 1, ds, am, pm));
     // last step's c2r: materialise V and max|V|, closing half kick; psi
     // comes back from its tiles into the row layout
@@ -882,6 +887,64 @@ static qx_status leapfrog_impl(qx_ctx *c, double ds, int nsteps, int first_half,
 qx_status qx_leapfrog(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)
 {
     return leapfrog_impl(c, ds, nsteps, first_half, last_half, true);
+}
+
+// steps whose drift tables share one upload (and one stream wait)
+static constexpr int SCHED_CHUNK = 32;
+
+qx_status qx_leapfrog_schedule(qx_ctx *c, int nsteps, const double *drift_dt, const double *kick_dt,
+                               const double *pref)
+{
+    CHECK_CTX(c);
+    if (nsteps < 1) return fail(QX_EINVAL, "nsteps must be >= 1");
+    if (!drift_dt || !kick_dt || !pref) return fail(QX_EINVAL, "null schedule array");
+    for (int j = 0; j <= nsteps; ++j) {
+        if (!std::isfinite(kick_dt[j])) return fail(QX_EINVAL, "kick_dt must be finite");
+        if (j < nsteps && !(std::isfinite(drift_dt[j]) && std::isfinite(pref[j])))
+            return fail(QX_EINVAL, "drift_dt and pref must be finite");
+    }
+    const int N = c->N;
+    const size_t step_bytes = (sizeof(double2) + sizeof(float2)) * (size_t)N;
+    TRY(ensure(c, (void **)&c->sched, step_bytes * SCHED_CHUNK));
+    const bool t = tiled(c);
+    if (t) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
+    if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
+    const MulArgs pm = poisson_args(c);
+    const int np = step_pass_count(c);
+    const long long rpm = c->dim == 3 ? N : c->rows;
+    std::vector<unsigned char> host(step_bytes * SCHED_CHUNK);
+    std::vector<double2> T(N);
+    for (int j0 = 0; j0 < nsteps; j0 += SCHED_CHUNK) {
+        const int m = std::min(SCHED_CHUNK, nsteps - j0);
+        for (int i = 0; i < m; ++i) {
+            fill_phase_table(T.data(), N, c->L, drift_dt[j0 + i]);
+            unsigned char *h = host.data() + step_bytes * i;
+            std::memcpy(h, T.data(), sizeof(double2) * N);
+            fill_phase_m1((float2 *)(h + sizeof(double2) * N), T.data(), N);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));  // the previous chunk's passes are done with the tables
+        HIPCHK(hipMemcpy(c->sched, host.data(), step_bytes * m, hipMemcpyHostToDevice));
+        for (int i = 0; i < m; ++i) {
+            const int j = j0 + i;
+            MulArgs am = axis_drift_args(c);
+            am.ptab = (const double2 *)(c->sched + step_bytes * i);
+            am.ftab = (const float2 *)(c->sched + step_bytes * i + sizeof(double2) * N);
+            if (j == 0)  // the opening kick reads the stored V
+                c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, rpm, kick_dt[0], nullptr, C2R_KICK | C2R_DRIFT,
+                                 nullptr, c->stream, nullptr, am.ptab, am.ftab, 0, 1.0 / N);
+            else
+                step_pass(c, 0, kick_dt[j], pref[j], c->psi, c->psit, c->vt, c->vtk, am, pm);
+            for (int p = 1; p < np; ++p)
+                step_pass(c, p, kick_dt[j], pref[j], c->psi, c->psit, c->vt, c->vtk, am, pm);
+        }
+    }
+    // closing c2r: V and max|V| of the last step, its kick, psi back to rows
+    TRY(reset_vmax(c));
+    const double kl = kick_dt[nsteps];
+    c->eng->c2r_rows(c->vt, c->V, c->psi, c->rows, rpm, kl, nullptr,
+                     C2R_SRC_VT | C2R_STORE_V | C2R_VMAX | (kl != 0.0 ? C2R_KICK : 0) | (t ? C2R_SRC_TILES : 0),
+                     c->vmax_bits, c->stream, c->psit);
+    return read_vmax(c);
 }
 
 
@@ -966,15 +1029,15 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     if (tiled(c)) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
     // pass 0 of a tiled lattice reads psi from its tiles: fill them first
-    if (tiled(c)) step_pass(c, 1, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
+    if (tiled(c)) step_pass(c, 1, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     qx_status st = QX_OK;
     for (int p = 0; p < np && st == QX_OK; ++p) {
-        step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);  // warm
+        step_pass(c, p, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);  // warm
         if (hipEventRecord(e0, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
-        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
+        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
         if (hipEventRecord(e1, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
         float t = 0.f;
         if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <iostream>
+#include <vector>
 
 void defaults(Params &p)  // src/parse.cxx:7-29
 {
@@ -223,13 +224,27 @@ void Field::propagate()  // src/field.cxx:388-402
 void Field::leapfrog(int nsteps, bool first_half, bool last_half)
 {
     if (cosmo_ != 0) {
-        // ds and pref change every step: run the operators one by one
-        if (first_half) half_kick(); else full_kick();
-        for (int i = 0; i < nsteps; ++i) {
-            drift_update();
-            if (i != nsteps - 1) full_kick();
+        if (nsteps == 0) {
+            if (first_half) half_kick(); else full_kick();
+            if (last_half) half_kick();
+            return;
         }
-        if (last_half) half_kick();
+        // ds and a change after every drift (updateTime): unroll the
+        // schedule on the host and run it as one fused loop
+        std::vector<double> drift_dt(nsteps), kick_dt(nsteps + 1), pref(nsteps);
+        kick_dt[0] = first_half ? 0.5 * ds_ : ds_;
+        for (int i = 0; i < nsteps; ++i) {
+            drift_dt[i] = ds_;
+            updateTime();
+            pref[i] = norm_ * a_;
+            kick_dt[i + 1] = i != nsteps - 1 ? ds_ : (last_half ? 0.5 * ds_ : 0.0);
+        }
+        syncToDevice();
+        setPhysics();
+        qx_throw(qx_leapfrog_schedule(ctx_, nsteps, drift_dt.data(), kick_dt.data(), pref.data()));
+        Vmax_ = qx_vmax(ctx_);
+        psi_host_valid_ = false;
+        V_host_valid_ = false;
         return;
     }
     syncToDevice();

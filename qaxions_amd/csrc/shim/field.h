@@ -60,6 +60,7 @@ class Field {
     int curr() const { return curr_; }
     double time() const { return s_; }
     double a() const { return a_; }
+    double ds() const { return ds_; }
     int nsteps() const { return nsteps_; }
     int cosmo() const { return cosmo_; }
 

@@ -2,14 +2,17 @@
 // (src/examples/fieldtest.cxx:11-105): each entry point builds a Field from
 // defaults() with the soliton IC, runs one operator sequence and returns one
 // real scalar.  `fp32` (keyword, default False) selects complex64 device state.
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include "field.h"
 #include "ic.h"
 
+#include <complex>
+
 namespace py = pybind11;
 
 namespace {
-std::unique_ptr<Field> make_field(int n, int N, bool fp32)
+std::unique_ptr<Field> make_field(int n, int N, bool fp32, bool mre = false)
 {
     Params pars;
     defaults(pars);
@@ -18,6 +21,7 @@ std::unique_ptr<Field> make_field(int n, int N, bool fp32)
     pars.norm = 1;
     pars.verb = false;
     pars.precision = fp32 ? Precision::FP32 : Precision::FP64;
+    if (mre) pars.cosmotype = CosmoType::MRE;
     auto field = std::make_unique<Field>(pars);
     auto ic = createIC(pars.ictype, pars);
     ic->apply(*field);
@@ -58,6 +62,32 @@ double test_kick_drift_step(int n, int N, double dt, int idx, bool fp32)
     return field->getPsiIdx(idx);
 }
 
+// Not in the reference module: an MRE run of the time loop
+// (src/main.cxx:65-85) from the soliton IC, either as Field::leapfrog's one
+// fused call (fused) or operator by operator as the reference's loop calls
+// them.  Returns (psi, a, ds, s) after nsteps.
+py::tuple mre_leapfrog(int n, int N, int nsteps, bool fused, bool fp32)
+{
+    auto field = make_field(n, N, fp32, true);
+    field->updatePotential();
+    if (fused) {
+        field->leapfrog(nsteps, true, true);
+    } else {
+        field->half_kick();
+        for (int i = 0; i < nsteps; ++i) {
+            field->drift_update();
+            if (i != nsteps - 1) field->full_kick();
+        }
+        field->half_kick();
+    }
+    const size_t S = field->sites();
+    py::array_t<std::complex<double>> out(S);
+    const qx_complex *p = field->psi();
+    auto *o = out.mutable_data();
+    for (size_t i = 0; i < S; ++i) o[i] = std::complex<double>(p[i][0], p[i][1]);
+    return py::make_tuple(out, field->a(), field->ds(), field->time());
+}
+
 PYBIND11_MODULE(fieldtest, m)
 {
     m.def("test_kick", &test_kick, "", py::arg("n"), py::arg("N"), py::arg("dt"), py::arg("idx"),
@@ -68,4 +98,6 @@ PYBIND11_MODULE(fieldtest, m)
           py::arg("fp32") = false);
     m.def("test_kick_drift_step", &test_kick_drift_step, "", py::arg("n"), py::arg("N"),
           py::arg("dt"), py::arg("idx"), py::arg("fp32") = false);
+    m.def("mre_leapfrog", &mre_leapfrog, "", py::arg("n"), py::arg("N"), py::arg("nsteps"),
+          py::arg("fused") = true, py::arg("fp32") = false);
 }

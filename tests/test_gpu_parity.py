@@ -225,6 +225,68 @@ def test_leapfrog_segments_and_propagate(qx, prec):
     assert rel_l2(d.psi, o.kick(o.kick(psi, V, ds / 2), V, ds / 2)) < TOL[prec]
 
 
+MRE_RHO_BAR = 1.5 * (2 * math.pi) ** 2 * 1.27 ** 2   # src/field.cxx:86
+
+
+@pytest.mark.parametrize("dim,N,L,steps", [(2, 64, 10.0, 100), (3, 8, 1.0, 40), (3, 32, 20.0, 40)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_mre_schedule_matches_oracle(qx, dim, N, L, steps, prec, tol):
+    """The MRE run (src/field.cxx:82-89, 349-361; a_i = 0.1, dt = 1e-4): ds
+    and pref = a change every step.  More steps than one table upload holds
+    (32), untiled (8^3) and tiled (32^3) 3D."""
+    dd, kk, pp, _, _ = o.mre_schedule(0.1, 1e-4, steps)
+    psi = o.ic_grf(dim, N, L, 21)
+    V, _, _ = o.update_potential(psi, L, 0.1, MRE_RHO_BAR)
+    want, Vw = o.leapfrog_schedule(psi, V, L, dd, kk, pp, rho_bar=MRE_RHO_BAR)
+    lat = lattice(qx, dim, N, L, prec, pref=0.1, rho_bar=MRE_RHO_BAR)
+    lat.psi = psi
+    lat.update_potential()
+    lat.leapfrog_schedule(dd, kk, pp)
+    err = rel_l2(lat.psi, want)
+    print(f"psi rel L2 after {steps} MRE steps ({prec}): {err:.3e}")
+    assert err <= tol
+    assert rel_l2(lat.V, Vw) <= 10 * tol
+    assert lat.vmax == pytest.approx(np.abs(Vw).max(), rel=10 * tol)
+
+
+@pytest.mark.parametrize("dim,N,prec", [(2, 64, "fp64"), (3, 32, "fp32"), (3, 8, "fp32")])
+def test_constant_schedule_is_leapfrog_bitwise(qx, dim, N, prec):
+    """Constant per-step values run the same kernels with the same tables as
+    qx_leapfrog (graph-captured there for these sizes)."""
+    L, ds, ns = 10.0, 1e-3, 20
+    psi = o.ic_grf(dim, N, L, 22)
+    a = lattice(qx, dim, N, L, prec, pref=1.0)
+    a.psi = psi
+    a.update_potential()
+    b = lattice(qx, dim, N, L, prec, pref=1.0)
+    b.psi = psi
+    b.update_potential()
+    a.leapfrog(ds, ns)
+    b.leapfrog_schedule([ds] * ns, [ds / 2] + [ds] * (ns - 1) + [ds / 2], [1.0] * ns)
+    assert np.array_equal(a.psi, b.psi)
+    assert np.array_equal(a.V, b.V)
+    with pytest.raises(qx.QxError):
+        b.leapfrog_schedule([], [ds], [])
+    with pytest.raises(qx.QxError):
+        b.leapfrog_schedule([ds], [ds, float("nan")], [1.0])
+
+
+@pytest.mark.parametrize("n,N,fp32", [(2, 32, False), (3, 16, False), (3, 16, True)])
+def test_field_mre_leapfrog_fused_equals_operators(n, N, fp32, in_golden_dir):
+    """Field::leapfrog in an MRE run (one fused call) against the reference's
+    operator-by-operator loop on the same Field; a, ds and s against the
+    oracle's unrolled schedule."""
+    import qaxions_amd.fieldtest as field
+    print(f"[case] mre n={n} N={N} fp32={fp32}", flush=True)
+    steps = 36
+    pf, af, dsf, sf = field.mre_leapfrog(n, N, steps, fused=True, fp32=fp32)
+    po, ao, dso, so = field.mre_leapfrog(n, N, steps, fused=False, fp32=fp32)
+    _, kk, pp, a, s = o.mre_schedule(0.1, 1e-4, steps)
+    assert af == ao == a and sf == so == s
+    assert dsf == dso and dsf == pytest.approx(2 * kk[-1], rel=1e-15)
+    assert rel_l2(pf, po) < (1e-12 if not fp32 else 1e-5)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
 def test_plain_method_256sq_100_steps(qx, prec, tol):
     """2D 256^2, L = 10, dt = 1e-3, 100 KDK steps, GRF IC: psi relative L2

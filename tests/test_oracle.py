@@ -185,6 +185,65 @@ def test_leapfrog_segments_chain():
     assert np.abs(a - b).max() < 1e-14
 
 
+MRE_RHO_BAR = 1.5 * (2 * math.pi) ** 2 * 1.27 ** 2   # src/field.cxx:86
+
+
+def test_mre_schedule_recursion_and_continuum_limit():
+    """Field::setCosmo MRE + updateTime (src/field.cxx:82-89, 349-361),
+    unrolled.  Checked in forms other than the oracle's: every step advances
+    conformal time a ds by exactly 2.5e-4, so a grows by C sqrt(1 + a) 2.5e-4
+    (C = 1.27^2); the continuum solution of da/dtau = C sqrt(1 + a) is
+    a(tau) = (sqrt(1 + a_i) + C tau / 2)^2 - 1, and forward Euler on a convex
+    solution lags it by about tau h C^2 / 4."""
+    ai, dt, n = 0.1, 1e-4, 1000
+    dd, kk, pp, a, s = o.mre_schedule(ai, dt, n)
+    C, h = 1.27 ** 2, 2.5e-4
+    assert dd[0] == dt / ai and kk[0] == 0.5 * dt / ai          # ds = dt / a_i, half opening kick
+    a_prev = np.concatenate([[ai], pp[:-1]])                      # a before each updateTime
+    assert np.allclose(pp - a_prev, C * h * np.sqrt(1 + a_prev), rtol=1e-10, atol=0)
+    assert np.allclose(dd[1:] * a_prev[:-1], h, rtol=1e-14, atol=0)   # ds = 2.5e-4 / a
+    assert np.array_equal(kk[1:-1], dd[1:])                       # merged kicks use the new ds
+    assert kk[-1] == pytest.approx(0.5 * h / pp[-2], rel=1e-15)   # closing half kick
+    assert a == pp[-1]
+    assert s == pytest.approx(np.sum(h / a_prev), rel=1e-13)     # s sums the new ds
+    tau = n * h
+    exact = (math.sqrt(1 + ai) + C * tau / 2) ** 2 - 1
+    lag = exact - a
+    bound = tau * h * C * C / 4
+    assert 0.5 * bound < lag < 1.5 * bound
+    # the segment flags
+    _, kk2, _, _, _ = o.mre_schedule(ai, dt, 3, first_half=0, last_half=0)
+    assert kk2[0] == dt / ai and kk2[-1] == 0.0
+
+
+def test_leapfrog_schedule_constant_equals_leapfrog():
+    """With constant per-step values the scheduled loop is the pinned
+    merged-kick loop, bit for bit."""
+    N, L, dt, pref = 16, 10.0, 1e-2, 1.3
+    psi = o.ic_grf(2, N, L, 8)
+    V, _, _ = o.update_potential(psi, L, pref)
+    a, Va = o.leapfrog(psi, V, L, dt, 5, pref)
+    b, Vb = o.leapfrog_schedule(psi, V, L, [dt] * 5, [dt / 2] + [dt] * 4 + [dt / 2], [pref] * 5)
+    assert np.array_equal(a, b) and np.array_equal(Va, Vb)
+
+
+def test_leapfrog_schedule_is_the_operator_sequence():
+    """The scheduled loop composes the pinned operators in the reference's
+    order (src/main.cxx:65-85) with each step's own constants."""
+    N, L = 8, 1.0
+    psi = o.ic_grf(3, N, L, 4) * 0.5
+    dd, kk, pp, _, _ = o.mre_schedule(0.1, 1e-4, 4)
+    V, _, _ = o.update_potential(psi, L, 0.1, MRE_RHO_BAR)
+    got, Vg = o.leapfrog_schedule(psi, V, L, dd, kk, pp, rho_bar=MRE_RHO_BAR)
+    p = o.kick(psi, V, kk[0])
+    for j in range(4):
+        p = o.drift(p, L, dd[j])
+        V, _, _ = o.update_potential(p, L, pp[j], MRE_RHO_BAR)
+        p = o.kick(p, V, kk[j + 1])
+    assert np.abs(got - p).max() < 1e-14
+    assert np.abs(Vg - V).max() < 1e-12 * np.abs(V).max()
+
+
 def test_mass_conserved_energy_bounded():
     N, L, dt, pref = 32, 10.0, 1e-3, 1.0
     psi = o.ic_grf(2, N, L, 11)
