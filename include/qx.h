@@ -71,6 +71,23 @@ qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src);
 /* device pointers of psi / V (element type set by the precision) */
 qx_status qx_device_state(qx_ctx *ctx, void **psi, void **V);
 
+/* ---- asynchronous snapshots: the host reads of IO::writeConf
+ * (src/io/io.cxx:321-369, which takes V and psi from Field's host arrays)
+ * without stopping the time loop.  qx_snap_take queues device copies of
+ * psi and/or V on the context's stream and returns at once; their transfer
+ * to pinned host memory runs on the snapshot's own stream behind the steps
+ * queued after it.  qx_snap_wait blocks until that transfer is done and
+ * converts to complex128 / float64 (outputs nullable); it may be called
+ * from another thread than the one driving the context, and must be called
+ * before the next take.  A snapshot holds S complex and/or S real of device
+ * memory and as much pinned host memory; destroy it before its context. */
+typedef struct qx_snap qx_snap;
+enum { QX_SNAP_PSI = 1, QX_SNAP_V = 2 };
+qx_status qx_snap_create(qx_ctx *ctx, int what, qx_snap **out);
+qx_status qx_snap_take(qx_snap *snap);
+qx_status qx_snap_wait(qx_snap *snap, double *psi_c128, double *V);
+qx_status qx_snap_destroy(qx_snap *snap);
+
 /* ---- split-step operators ------------------------------------------------ */
 /* Field::kick (src/field.cxx:93-116): psi <- psi exp(-i dt V) */
 qx_status qx_kick(qx_ctx *ctx, double dt);

@@ -59,6 +59,10 @@ SIGNATURES = {
     "qx_download_V": (C.c_int, [C.c_void_p, _dp]),
     "qx_copy_state": (C.c_int, [C.c_void_p, C.c_void_p]),
     "qx_device_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "qx_snap_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "qx_snap_take": (C.c_int, [C.c_void_p]),
+    "qx_snap_wait": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "qx_snap_destroy": (C.c_int, [C.c_void_p]),
     "qx_kick": (C.c_int, [C.c_void_p, C.c_double]),
     "qx_drift": (C.c_int, [C.c_void_p, C.c_double]),
     "qx_update_potential": (C.c_int, [C.c_void_p, _dp]),
@@ -398,6 +402,39 @@ class Lattice:
 
     def synchronize(self):
         _check(lib().qx_synchronize(self._h))
+
+
+class Snapshot:
+    """Asynchronous copy of a lattice's psi and/or V (qx_snap_*): take()
+    queues device copies in stream order and returns; wait() returns the
+    arrays as they were at take() while later steps run on."""
+
+    def __init__(self, lattice, psi=True, V=True):
+        what = (1 if psi else 0) | (2 if V else 0)
+        h = C.c_void_p()
+        _check(lib().qx_snap_create(lattice._h, what, C.byref(h)))
+        self._h, self._lat, self._psi, self._V = h, lattice, bool(psi), bool(V)
+
+    def take(self):
+        _check(lib().qx_snap_take(self._h))
+
+    def wait(self):
+        psi = np.empty(self._lat.shape, np.complex128) if self._psi else None
+        V = np.empty(self._lat.shape, np.float64) if self._V else None
+        _check(lib().qx_snap_wait(self._h, None if psi is None else _p(psi.view(np.float64)),
+                                  None if V is None else _p(V)))
+        return psi, V
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qx_snap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def blob_table(nblobs, seed, L):

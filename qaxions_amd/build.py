@@ -3,6 +3,9 @@
   libqx.so          HIP kernels + C ABI (include/qx.h), hipcc --offload-arch=gfx950
   libqxshim.so      C++ Field/FFTBackend/Spectrum mirror of the reference, over the C ABI
   fieldtest*.so     pybind11 module with the reference's fieldtest entry points
+  libqxio.so        HDF5 configuration files (IO, SnapshotWriter) and measure(),
+                    built when an HDF5 C library is found (HDF5_DIR, default /opt/conda)
+  h5test*.so        pybind11 module with the reference's h5test entry points
 
 Run ``python -m qaxions_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -27,6 +30,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 LIB = os.path.join(HERE, "libqx.so")
 SHIM = os.path.join(HERE, "libqxshim.so")
+IOLIB = os.path.join(HERE, "libqxio.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 
 
 def _run(cmd):
@@ -65,7 +70,7 @@ def build_shim():
     import pybind11
     inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(CSRC, "shim")]
     shim_srcs = [os.path.join(CSRC, "shim", f) for f in
-                 ("field.cxx", "spectrum.cxx", "solitons.cxx", "meas.cxx", "utils.cxx")]
+                 ("field.cxx", "spectrum.cxx", "solitons.cxx", "utils.cxx")]
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc, *shim_srcs,
           "-L", HERE, "-lqx", "-Wl,-rpath,$ORIGIN", "-o", SHIM + ".tmp"])
     os.replace(SHIM + ".tmp", SHIM)
@@ -79,6 +84,34 @@ def build_shim():
     return SHIM, mod
 
 
+def hdf5_found() -> bool:
+    return (os.path.isfile(os.path.join(HDF5_DIR, "include", "hdf5.h")) and
+            os.path.isfile(os.path.join(HDF5_DIR, "lib", "libhdf5.so")))
+
+
+def build_io():
+    """libqxio.so and the h5test module.  libqxio's RUNPATH names the HDF5
+    directory after $ORIGIN; the C++ runtime it needs is the one its loader
+    (libqxshim, h5test or a program) has mapped already."""
+    import pybind11
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(CSRC, "shim"),
+           "-I", os.path.join(HDF5_DIR, "include")]
+    h5lib = os.path.join(HDF5_DIR, "lib")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc,
+          os.path.join(CSRC, "shim", "io.cxx"), os.path.join(CSRC, "shim", "meas.cxx"),
+          "-L", HERE, "-lqxshim", "-lqx", "-L", h5lib, "-lhdf5", "-pthread",
+          "-Wl,-rpath,$ORIGIN:" + h5lib, "-o", IOLIB + ".tmp"])
+    os.replace(IOLIB + ".tmp", IOLIB)
+    ext = sysconfig.get_config_var("EXT_SUFFIX")
+    mod = os.path.join(HERE, "h5test" + ext)
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc,
+          "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+          os.path.join(CSRC, "shim", "h5test.cxx"),
+          "-L", HERE, "-lqxio", "-lqxshim", "-lqx", "-Wl,-rpath,$ORIGIN", "-o", mod + ".tmp"])
+    os.replace(mod + ".tmp", mod)
+    return IOLIB, mod
+
+
 def build(verbose: bool = True):
     lib = build_lib()
     if verbose:
@@ -87,6 +120,13 @@ def build(verbose: bool = True):
     if verbose:
         print("built", shim)
         print("built", mod)
+    if hdf5_found():
+        iolib, h5mod = build_io()
+        if verbose:
+            print("built", iolib)
+            print("built", h5mod)
+    elif verbose:
+        print("HDF5 not found under", HDF5_DIR, "- libqxio.so (IO, measure) not built")
 
 
 if __name__ == "__main__":

@@ -27,10 +27,12 @@
 #include "qx_engine.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace qx;
@@ -664,6 +666,118 @@ qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src)
     HIPCHK(hipMemcpyAsync(dst->vt, src->vt, (size_t)src->H * src->csz, hipMemcpyDeviceToDevice, dst->stream));
     dst->vmax = src->vmax;
     return sync(dst);
+}
+
+// ---- asynchronous snapshots ----------------------------------------------------
+struct qx_snap {
+    qx_ctx *ctx = nullptr;
+    int what = 0;
+    long long S = 0;
+    size_t csz = 0, rsz = 0;
+    void *dpsi = nullptr, *dV = nullptr;  // device copies taken on the context's stream
+    void *hpsi = nullptr, *hV = nullptr;  // pinned host copies, device precision
+    hipStream_t copy = nullptr;
+    hipEvent_t taken = nullptr, done = nullptr;
+    std::atomic<bool> pending{false};
+};
+
+qx_status qx_snap_destroy(qx_snap *s)
+{
+    if (!s) return QX_OK;
+    (void)hipSetDevice(s->ctx->device);
+    if (s->copy) (void)hipStreamSynchronize(s->copy);
+    for (void *b : {s->dpsi, s->dV})
+        if (b) (void)hipFree(b);
+    for (void *b : {s->hpsi, s->hV})
+        if (b) (void)hipHostFree(b);
+    if (s->taken) (void)hipEventDestroy(s->taken);
+    if (s->done) (void)hipEventDestroy(s->done);
+    if (s->copy) (void)hipStreamDestroy(s->copy);
+    delete s;
+    return QX_OK;
+}
+
+qx_status qx_snap_create(qx_ctx *c, int what, qx_snap **out)
+{
+    if (!out) return fail(QX_EINVAL, "out is null");
+    *out = nullptr;
+    CHECK_CTX(c);
+    if (what < 1 || what > 3) return fail(QX_EINVAL, "what: 1 psi, 2 V, 3 both");
+    std::unique_ptr<qx_snap, qx_status (*)(qx_snap *)> s(new qx_snap(), qx_snap_destroy);
+    s->ctx = c;
+    s->what = what;
+    s->S = c->S;
+    s->csz = c->csz;
+    s->rsz = c->rsz;
+    if (what & QX_SNAP_PSI) {
+        TRY(ensure(c, &s->dpsi, (size_t)c->S * c->csz));
+        HIPCHK(hipHostMalloc(&s->hpsi, (size_t)c->S * c->csz, hipHostMallocDefault));
+    }
+    if (what & QX_SNAP_V) {
+        TRY(ensure(c, &s->dV, (size_t)c->S * c->rsz));
+        HIPCHK(hipHostMalloc(&s->hV, (size_t)c->S * c->rsz, hipHostMallocDefault));
+    }
+    HIPCHK(hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s->taken, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
+    *out = s.release();
+    return QX_OK;
+}
+
+qx_status qx_snap_take(qx_snap *s)
+{
+    if (!s) return fail(QX_EINVAL, "null snapshot");
+    qx_ctx *c = s->ctx;
+    CHECK_CTX(c);
+    if (s->pending.load()) return fail(QX_ESTATE, "the previous snapshot has not been waited for");
+    // device copies in stream order: the steps queued after this call do
+    // not wait for the transfer to the host
+    if (s->dpsi) HIPCHK(hipMemcpyAsync(s->dpsi, c->psi, (size_t)s->S * s->csz, hipMemcpyDeviceToDevice, c->stream));
+    if (s->dV) HIPCHK(hipMemcpyAsync(s->dV, c->V, (size_t)s->S * s->rsz, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipEventRecord(s->taken, c->stream));
+    HIPCHK(hipStreamWaitEvent(s->copy, s->taken, 0));
+    if (s->dpsi) HIPCHK(hipMemcpyAsync(s->hpsi, s->dpsi, (size_t)s->S * s->csz, hipMemcpyDeviceToHost, s->copy));
+    if (s->dV) HIPCHK(hipMemcpyAsync(s->hV, s->dV, (size_t)s->S * s->rsz, hipMemcpyDeviceToHost, s->copy));
+    HIPCHK(hipEventRecord(s->done, s->copy));
+    s->pending.store(true);
+    return QX_OK;
+}
+
+// float -> double over host threads (a 512^3 snapshot is 0.4 G values)
+static void widen(const float *src, double *dst, long long n)
+{
+    const long long per = 1ll << 22;
+    const int nt = (int)std::max(1ll, std::min<long long>(std::max(1u, std::thread::hardware_concurrency()) / 2,
+                                                            std::min(16ll, (n + per - 1) / per)));
+    auto part = [&](int t) {
+        const long long a = n * t / nt, b = n * (t + 1) / nt;
+        for (long long i = a; i < b; ++i) dst[i] = (double)src[i];
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(part, t);
+    part(0);
+    for (auto &th : pool) th.join();
+}
+
+qx_status qx_snap_wait(qx_snap *s, double *psi_c128, double *V)
+{
+    if (!s) return fail(QX_EINVAL, "null snapshot");
+    if (!s->pending.load()) return fail(QX_ESTATE, "no snapshot taken");
+    if ((psi_c128 && !s->hpsi) || (V && !s->hV)) return fail(QX_EINVAL, "that array was not part of the snapshot");
+    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(hipEventSynchronize(s->done));
+    const long long S = s->S;
+    const bool f64 = s->rsz == sizeof(double);
+    if (psi_c128) {
+        if (f64) std::memcpy(psi_c128, s->hpsi, (size_t)S * 2 * sizeof(double));
+        else widen((const float *)s->hpsi, psi_c128, 2 * S);
+    }
+    if (V) {
+        if (f64) std::memcpy(V, s->hV, (size_t)S * sizeof(double));
+        else widen((const float *)s->hV, V, S);
+    }
+    s->pending.store(false);
+    return QX_OK;
 }
 
 qx_status qx_device_state(qx_ctx *c, void **psi, void **V)

@@ -1,0 +1,118 @@
+"""HDF5 configuration files of device Fields (libqxio.so through the
+h5test module): the reference's two HDF5 tests (tests/test_hdf5.py),
+background snapshots taken while the time loop runs on, restart from a
+file, and measure()'s SPECTRUM / RHO_GRID output (src/meas.cxx:13-63)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+JAXIONS_NORM = 0.01 ** 1.5 / 1.41421356237   # src/io/io.cxx:12,264-265
+
+
+@pytest.fixture(scope="module")
+def h5():
+    try:
+        import qaxions_amd.h5test as m
+    except ImportError as e:
+        pytest.fail(f"libqxio.so / h5test not built: {e}")
+    return m
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2, 3])
+def test_read_write(h5, idx, in_golden_dir, tmp_path):
+    """tests/test_hdf5.py:11-15: soliton Field -> writeConf(psi) -> readConf."""
+    print(f"[case] read_write idx={idx}", flush=True)
+    assert h5.test_read_write(idx, str(tmp_path / "test_rw.h5")) < 1e-10
+
+
+@pytest.mark.parametrize("idx", [1, 9])
+def test_read_jaxions(h5, idx, tmp_path, monkeypatch):
+    """tests/test_hdf5.py:17-28 with a synthetic ./jaxions_2D.hdf5 (the
+    reference ships none); expected value from the reference's reader
+    (ai = 0.01, see tests/test_io.py)."""
+    rng = np.random.default_rng(idx)
+    N = 32
+    m, v = rng.standard_normal(N * N), rng.standard_normal(N * N)
+    h5.write_jaxions_arrays(str(tmp_path / "jaxions_2D.hdf5"), N, 2, 10.0, m, v)
+    monkeypatch.chdir(tmp_path)
+    print(f"[case] read_jaxions idx={idx}", flush=True)
+    assert h5.test_read_jaxions(idx) == pytest.approx(m[idx] * JAXIONS_NORM, rel=1e-12)
+
+
+@pytest.mark.parametrize("n,N,fp32", [(2, 64, False), (2, 64, True), (3, 32, False), (3, 32, True)])
+def test_background_snapshot_and_restart(h5, n, N, fp32, in_golden_dir, tmp_path):
+    """The file holds the state at the snapshot although the loop ran on
+    before the transfer finished; a run restarted from it ends where the
+    uninterrupted run ends."""
+    print(f"[case] snapshot n={n} N={N} fp32={fp32}", flush=True)
+    f = str(tmp_path / "snap.h5")
+    out = h5.snapshot_restart(n, N, 24, 10, f, fp32=fp32)
+    hdr, psi = h5.read_conf_arrays(f)
+    assert hdr["N"] == N and hdr["dim"] == n and hdr["step"] == 10
+    assert hdr["time"] == pytest.approx(10 * 1e-4, rel=1e-12)   # defaults(): dt = 1e-4, STATIC
+    assert np.array_equal(psi, out["psi_at_snapshot"])
+    # rho/data holds V as the Field held it (writeConf, src/io/io.cxx:330-339)
+    assert not np.array_equal(out["psi_final"], out["psi_at_snapshot"])
+    assert rel_l2(out["psi_restart_final"], out["psi_final"]) < (2e-6 if fp32 else 1e-12)
+
+
+@pytest.mark.parametrize("legacy,background", [(True, False), (True, True), (False, False), (False, True)])
+def test_measure_writes_spectrum_and_grid(h5, legacy, background, in_golden_dir, tmp_path):
+    print(f"[case] measure legacy={legacy} background={background}", flush=True)
+    n, N = 2, 32
+    psi = h5.measure_run(n, N, str(tmp_path), 3, legacy_clobber=legacy, background=background)
+    spec = tmp_path / "spec_0003.txt"
+    grid = tmp_path / "field_0003.h5"
+    assert spec.exists() and grid.exists()
+    lines = spec.read_text().splitlines()
+    assert lines[0].startswith("# a:")
+    rows = np.array([[float(x) for x in l.split()] for l in lines[2:]])
+    assert rows.shape[1] == 3 and np.all(rows[:, 2] > 0)
+    hdr, _ = h5.read_conf_arrays(str(grid), with_psi=False)
+    assert hdr["step"] == 1 and hdr["N"] == N
+    rho = np.abs(psi) ** 2
+    got = h5.read_rho(str(grid))
+    if legacy:   # V after computeEnergy: |psi|^2 formed on the device
+        assert rel_l2(got, rho) < 1e-14
+    else:        # formed on the host from psi
+        assert np.array_equal(got, psi.real * psi.real + psi.imag * psi.imag)
+
+
+@pytest.mark.parametrize("dim,N,prec", [(3, 64, "fp32"), (2, 256, "fp64")])
+def test_snapshot_abi(qx, dim, N, prec):
+    """qx_snap_*: the arrays at take() while the loop runs on; one pending
+    snapshot at a time; what was not captured cannot be asked for."""
+    print(f"[case] snap dim={dim} N={N} {prec}", flush=True)
+    lat = qx.Lattice(dim, N, 10.0, precision=prec)
+    lat.ic_grf(3)
+    lat.update_potential()
+    lat.leapfrog(1e-3, 3, True, False)
+    psi0, V0 = lat.psi, lat.V
+    s = qx.Snapshot(lat)
+    s.take()
+    with pytest.raises(qx.QxError):
+        s.take()
+    lat.leapfrog(1e-3, 5, False, True)
+    psi, V = s.wait()
+    assert np.array_equal(psi, psi0) and np.array_equal(V, V0)
+    assert not np.array_equal(lat.psi, psi0)
+    with pytest.raises(qx.QxError):
+        s.wait()
+    s.take()
+    psi, V = s.wait()
+    assert np.array_equal(psi, lat.psi) and np.array_equal(V, lat.V)
+    v_only = qx.Snapshot(lat, psi=False)
+    v_only.take()
+    p, V = v_only.wait()
+    assert p is None and np.array_equal(V, lat.V)
+    with pytest.raises(qx.QxError):
+        qx.Snapshot(lat, psi=False, V=False)
+    s.close()
+    v_only.close()
