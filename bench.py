@@ -147,6 +147,30 @@ def main():
     r1, m1 = lat.density_stats()
     print(f"[bench] {ms_step:.4f} ms/step (events), wall {wall*1e3/args.steps:.4f} ms/step, "
           f"mass drift {abs(m1-m0)/m0:.2e}", file=sys.stderr, flush=True)
+    # four more segments of the same length, untimed by the contract's
+    # clock: the spread and the median of five
+    seg = [ms_step]
+    for _ in range(4):
+        lat.leapfrog(dt, 1, first_half=True, last_half=False)
+        ev.record(e0, stream)
+        lat.leapfrog(dt, args.steps, first_half=False, last_half=True)
+        ev.record(e1, stream)
+        seg.append(ev.elapsed_ms(e0, e1) / args.steps)
+    seg_sorted = sorted(seg)
+    print(f"[bench] five segments: {' '.join(f'{x:.4f}' for x in seg)} ms/step, median {seg_sorted[2]:.4f}",
+          file=sys.stderr, flush=True)
+
+    # the measurement between steps, timed on its own: P(k) of |psi|^2 in
+    # linear bins of `spb` shells (128 bins at 512^3), every 10 steps
+    nbins = max(1, (N // 2) // 2)
+    lat.spectrum(source="rho", nbins=nbins, spb=2)
+    reps_pk = 5
+    t0 = time.perf_counter()
+    for _ in range(reps_pk):
+        lat.spectrum(source="rho", nbins=nbins, spb=2)
+    ms_pk = (time.perf_counter() - t0) * 1e3 / reps_pk
+    print(f"[bench] P(k) in {nbins} bins: {ms_pk:.3f} ms (wall, incl. readback); every 10 steps adds "
+          f"{100 * ms_pk / (10 * ms_step):.1f}%", file=sys.stderr, flush=True)
 
     # per-pass live timing: the dominant pass and its roofline
     names, pms, pbytes = lat.time_step_passes(dt, reps=args.pass_reps)
@@ -188,8 +212,11 @@ def main():
                           "physical_bytes_per_site": sum(pbytes) / S,
                           "physical_frac": sum(pbytes) / (ms_step * 1e-3) / HBM_PEAK},
         "passes": {n_: {"us": t_ * 1e3, "GBps": b_ / t_ / 1e6} for n_, t_, b_ in zip(names, pms, pbytes)},
+        "spectrum": {"nbins": nbins, "shells_per_bin": 2, "ms": ms_pk,
+                     "overhead_every_10_steps": ms_pk / (10 * ms_step)},
         "mass_drift": abs(m1 - m0) / m0,
         "wall_ms_per_step": wall * 1e3 / args.steps,
+        "segments_ms_per_step": seg, "median_ms_per_step": seg_sorted[2],
     }
     if not args.no_cpu_baseline:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
