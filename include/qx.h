@@ -80,7 +80,8 @@ qx_status qx_device_state(qx_ctx *ctx, void **psi, void **V);
  * converts to complex128 / float64 (outputs nullable); it may be called
  * from another thread than the one driving the context, and must be called
  * before the next take.  A snapshot holds S complex and/or S real of device
- * memory and as much pinned host memory; destroy it before its context. */
+ * memory and as much pinned host memory.  Only take uses the context; wait
+ * and destroy remain valid after the context is destroyed. */
 typedef struct qx_snap qx_snap;
 enum { QX_SNAP_PSI = 1, QX_SNAP_V = 2 };
 qx_status qx_snap_create(qx_ctx *ctx, int what, qx_snap **out);

@@ -670,7 +670,8 @@ qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src)
 
 // ---- asynchronous snapshots ----------------------------------------------------
 struct qx_snap {
-    qx_ctx *ctx = nullptr;
+    qx_ctx *ctx = nullptr;  // dereferenced only by take
+    int device = 0;
     int what = 0;
     long long S = 0;
     size_t csz = 0, rsz = 0;
@@ -684,7 +685,9 @@ struct qx_snap {
 qx_status qx_snap_destroy(qx_snap *s)
 {
     if (!s) return QX_OK;
-    (void)hipSetDevice(s->ctx->device);
+    (void)hipSetDevice(s->device);
+    // a pending snapshot may still be copying on either stream
+    if (s->pending.load() && s->taken) (void)hipEventSynchronize(s->taken);
     if (s->copy) (void)hipStreamSynchronize(s->copy);
     for (void *b : {s->dpsi, s->dV})
         if (b) (void)hipFree(b);
@@ -705,6 +708,7 @@ qx_status qx_snap_create(qx_ctx *c, int what, qx_snap **out)
     if (what < 1 || what > 3) return fail(QX_EINVAL, "what: 1 psi, 2 V, 3 both");
     std::unique_ptr<qx_snap, qx_status (*)(qx_snap *)> s(new qx_snap(), qx_snap_destroy);
     s->ctx = c;
+    s->device = c->device;
     s->what = what;
     s->S = c->S;
     s->csz = c->csz;
@@ -764,7 +768,7 @@ qx_status qx_snap_wait(qx_snap *s, double *psi_c128, double *V)
     if (!s) return fail(QX_EINVAL, "null snapshot");
     if (!s->pending.load()) return fail(QX_ESTATE, "no snapshot taken");
     if ((psi_c128 && !s->hpsi) || (V && !s->hV)) return fail(QX_EINVAL, "that array was not part of the snapshot");
-    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipEventSynchronize(s->done));
     const long long S = s->S;
     const bool f64 = s->rsz == sizeof(double);

@@ -309,6 +309,7 @@ SnapshotWriter::SnapshotWriter(const Field &field, bool save_psi, bool rho_from_
 {
     const int what = ((save_psi || rho_from_psi) ? QX_SNAP_PSI : 0) | (rho_from_psi ? 0 : QX_SNAP_V);
     qx_throw(qx_snap_create(field.context(), what, &snap_));
+    ctx_ = field.context();
 }
 
 SnapshotWriter::~SnapshotWriter()
@@ -330,7 +331,8 @@ void SnapshotWriter::wait()
 void SnapshotWriter::write(const Field &field, const std::string &filename)
 {
     wait();
-    if (field.context() == nullptr) throw std::runtime_error("SnapshotWriter: Field has no device state");
+    if (field.context() != ctx_)
+        throw std::invalid_argument("SnapshotWriter: write() takes the Field the writer was made for");
     field.syncToDevice();
     qx_throw(qx_snap_take(snap_));
     const ConfHeader h{field.size(), field.dim(), field.curr(), field.time(), field.a(), field.Lbox()};

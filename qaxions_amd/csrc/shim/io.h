@@ -91,6 +91,7 @@ class SnapshotWriter {
 
   private:
     qx_snap *snap_ = nullptr;
+    const qx_ctx *ctx_ = nullptr;  // the Field's device state the snapshot copies
     bool save_psi_ = false;
     bool rho_from_psi_ = false;
     std::thread worker_;
