@@ -72,7 +72,7 @@ qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src);
 qx_status qx_device_state(qx_ctx *ctx, void **psi, void **V);
 
 /* ---- asynchronous snapshots: the host reads of IO::writeConf
- * (src/io/io.cxx:321-369, which takes V and psi from Field's host arrays)
+ * (src/io/io.cxx:314-370, which takes V and psi from Field's host arrays)
  * without stopping the time loop.  qx_snap_take queues device copies of
  * psi and/or V on the context's stream and returns at once; their transfer
  * to pinned host memory runs on the snapshot's own stream behind the steps
