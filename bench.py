@@ -84,6 +84,30 @@ def cpu_baseline(lat, N, L, dt, steps, threads):
     return N ** 3 * steps / t, t, np
 
 
+def fp32_vs_fp64(qx, N, dim, L, dt, steps):
+    """The headline workload stepped `steps` times from the same seeded IC in
+    an FP32 and an FP64 context: relative L2 of psi and each run's mass
+    drift.  The FP64 device path matches the FP64 oracle to 1e-12 at the
+    sizes the oracle can run, so it stands in for it at this size."""
+    import numpy as np
+    out = {}
+    psis = {}
+    for prec in ("fp64", "fp32"):
+        lat = qx.Lattice(dim, N, L, precision=prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.ic_grf(seed=0, amp=0.1)
+        lat.update_potential()
+        _, m0 = lat.density_stats()
+        lat.leapfrog(dt, steps)
+        _, m1 = lat.density_stats()
+        psis[prec] = lat.psi
+        out[f"mass_drift_{prec}"] = abs(m1 - m0) / m0
+        lat.close()
+    a, b = psis["fp32"], psis["fp64"]
+    out["rel_l2"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["steps"] = steps
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +119,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-against-FP64 run of the same workload")
     ap.add_argument("--pass-reps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-variant autotuner")
     ap.add_argument("--workload", default="kdk", choices=["kdk", "ensemble", "learned", "copy"],
@@ -218,6 +244,13 @@ def main():
         "wall_ms_per_step": wall * 1e3 / args.steps,
         "segments_ms_per_step": seg, "median_ms_per_step": seg_sorted[2],
     }
+    if not args.no_accuracy and args.precision == "fp32":
+        print(f"[bench] accuracy: {args.steps} steps in FP32 and FP64 from the same IC",
+              file=sys.stderr, flush=True)
+        acc = fp32_vs_fp64(qx, N, dim, L, dt, args.steps)
+        print(f"[bench] FP32 vs FP64 psi rel L2 {acc['rel_l2']:.3e}; mass drift FP32 "
+              f"{acc['mass_drift_fp32']:.2e}, FP64 {acc['mass_drift_fp64']:.2e}", file=sys.stderr, flush=True)
+        out["fp32_vs_fp64"] = acc
     if not args.no_cpu_baseline:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         print(f"[bench] cpu baseline: {args.cpu_steps} steps of the FP64 oracle on {threads} threads",

@@ -363,6 +363,30 @@ def test_config1_256cube_fp32_against_fp64(qx):
     assert drift["fp32"][0] < 1e-5 and drift["fp32"][1] < 1e-4
 
 
+def test_headline_512cube_fp32_against_fp64_100_steps(qx):
+    """The headline workload at full size (3D 512^3, L = 40, dt = 2.5e-4,
+    GRF seed 0): after 100 steps the FP32 state is within 1e-5 relative L2
+    of the FP64 run, whose path matches the FP64 oracle to 1e-12 at the
+    oracle's sizes; mass drift 1e-13 (FP64) and 1e-5 (FP32)."""
+    N, L, dt, steps = 512, 40.0, 2.5e-4, 100
+    out, drift = {}, {}
+    for prec in ("fp64", "fp32"):
+        print(f"[case] 512^3 {prec} {steps} steps", flush=True)
+        lat = qx.Lattice(3, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.ic_grf(0, 0.1)
+        lat.update_potential()
+        _, m0 = lat.density_stats()
+        lat.leapfrog(dt, steps)
+        _, m1 = lat.density_stats()
+        out[prec] = lat.psi
+        drift[prec] = abs(m1 - m0) / m0
+        lat.close()
+    err = rel_l2(out["fp32"], out["fp64"])
+    print(f"512^3 fp32 vs fp64 after {steps} steps: {err:.3e}; mass drift {drift}")
+    assert err <= 1e-5
+    assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-5
+
+
 # ------------------------------------------------------------------ initial conditions
 @pytest.mark.parametrize("dim,N", [(2, 64), (3, 16)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
