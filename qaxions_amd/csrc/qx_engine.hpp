@@ -114,8 +114,8 @@ template <int N, typename R> struct EngineT final : Engine {
                   const void *tiles, const double2 *ptab, const float2 *ftab, long long ptab_stride,
                   double inv_n) override
     {
-        const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
-        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
+        const unsigned blocks = (unsigned)((rows + G::TLC - 1) / G::TLC);
+        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS_C, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
                                                              (R)kick_dt, kick_member, flags, vmax,
                                                              (const C *)tw, (const C *)tiles, ptab, ftab,
                                                              ptab_stride, inv_n);

@@ -63,20 +63,31 @@ template <int N, typename R> struct Geom {
     static constexpr int NP = 1 << ilog2(NP3);
     static constexpr int TL = 2 * NP;                 // rows per block
     static constexpr int ROW_THREADS = NP * PR::TPL;
-    // c2r_rows transforms a pair's two psi rows in lockstep, each with its
-    // own exchange line, where two lines per pair fit in LDS beside the
-    // twiddles (else one after the other through one line)
-    static constexpr bool TWO_LINES =
-        (long long)(2 * NP * PR::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
     static constexpr int XCH1 = NP * PR::PADDED;                        // one line per pair
-    static constexpr int XCH = (TWO_LINES ? 2 : 1) * XCH1;             // c2r_rows' exchange slots
     // staging of the half spectrum, [k][row] at pitch TL + 1 (TL >= 8): a
     // wave reading one row at 64 consecutive k then spreads over all banks
     // (at pitch TL = 16, 128 bytes, every lane hits the same bank pair)
     static constexpr int STP = TL >= 8 ? TL + 1 : TL;
     static constexpr int STG = (N / 2 + 1) * STP;     // staging slots
-    static constexpr int ROW_LDS = XCH > STG ? XCH : STG;       // c2r_rows
     static constexpr int ROW_LDS1 = XCH1 > STG ? XCH1 : STG;    // r2c_rows
+
+    // c2r_rows.  Where a line is one wavefront (TPL = 64) the kernel is
+    // latency-limited and its registers allow 5 waves per SIMD: it takes
+    // blocks of 4 waves with one exchange line per pair (22 KB of LDS), so
+    // that 5 blocks fit a CU.  Otherwise it keeps the r2c_rows blocks and
+    // transforms a pair's two psi rows in lockstep, each with its own
+    // exchange line, where two lines per pair fit in LDS beside the twiddles
+    // (else one after the other through one line).
+    static constexpr bool WAVE_LINE = PR::TPL == 64;
+    static constexpr int NPC = (WAVE_LINE && NP > 4) ? 4 : NP;
+    static constexpr int TLC = 2 * NPC;               // rows per block
+    static constexpr int ROW_THREADS_C = NPC * PR::TPL;
+    static constexpr bool TWO_LINES =
+        !WAVE_LINE && (long long)(2 * NPC * PR::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
+    static constexpr int XCH = (TWO_LINES ? 2 : 1) * NPC * PR::PADDED;  // exchange slots
+    static constexpr int STPC = TLC >= 8 ? TLC + 1 : TLC;
+    static constexpr int STGC = (N / 2 + 1) * STPC;
+    static constexpr int ROW_LDS = XCH > STGC ? XCH : STGC;
 };
 
 // pointwise factor in fused passes
@@ -543,7 +554,7 @@ enum {
 };
 
 template <int N, typename R>
-__global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
+__global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS_C))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
          const double *__restrict__ kick_member, int flags,
@@ -554,7 +565,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     using C = typename cplx<R>::T;
     using P = typename Geom<N, R>::PR;
     using G = Geom<N, R>;
-    constexpr int TL = G::TL, H1 = N / 2 + 1;
+    constexpr int TL = G::TLC, H1 = N / 2 + 1;
     __shared__ C lds[G::ROW_LDS];
     __shared__ C tw[N];
     __shared__ double wmax[16];
@@ -585,18 +596,18 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         // stage [k][l] of the block's TL rows: all loads issued before the
         // first LDS store
         C *stg = lds;
-        constexpr int NS = (H1 * TL + G::ROW_THREADS - 1) / G::ROW_THREADS;
+        constexpr int NS = (H1 * TL + G::ROW_THREADS_C - 1) / G::ROW_THREADS_C;
         C tmp[NS];
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            const int i = threadIdx.x + j * G::ROW_THREADS;
+            const int i = threadIdx.x + j * G::ROW_THREADS_C;
             const int k = i / TL, l = i % TL;
             tmp[j] = (i < H1 * TL && r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
         }
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            const int i = threadIdx.x + j * G::ROW_THREADS;
-            if (i < H1 * TL) stg[(i / TL) * G::STP + i % TL] = tmp[j];
+            const int i = threadIdx.x + j * G::ROW_THREADS_C;
+            if (i < H1 * TL) stg[(i / TL) * G::STPC + i % TL] = tmp[j];
         }
         __syncthreads();
         C z[P::E];
@@ -605,7 +616,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             const int j = t + q * P::TPL;
             const bool lo = j <= N / 2;
             const int k = lo ? j : N - j;
-            C xa = stg[k * G::STP + 2 * p], xb = stg[k * G::STP + 2 * p + 1];
+            C xa = stg[k * G::STPC + 2 * p], xb = stg[k * G::STPC + 2 * p + 1];
             if (k == 0 || k == N / 2) { xa.y = 0; xb.y = 0; }
             // Z[j] = Xa[j] + i Xb[j] (j <= N/2), conj Xa[N-j] + i conj Xb[N-j]
             z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
@@ -661,7 +672,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         }
         if (flags & C2R_DRIFT) {
             const long long ma_ = (oka ? ra / rpm : 0) * ptab_stride, mb_ = (okb ? rb / rpm : 0) * ptab_stride;
-            C *mine_b = lds + (G::NP + p) * P::PADDED;
+            C *mine_b = lds + (G::NPC + p) * P::PADDED;
             __syncthreads();
             if constexpr (G::TWO_LINES) {
                 fft_line2<N, -1, LdsContig>(a, b, t, mine, mine_b, 1, tw);
