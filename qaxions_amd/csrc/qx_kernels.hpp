@@ -571,7 +571,9 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     __shared__ double wmax[16];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
-    const long long r0 = (long long)blockIdx.x * TL;
+    // with 8-row blocks a block reads half of each 128 B line of vt: the
+    // block holding the other half runs next on the same XCD
+    const long long r0 = xcd_block() * TL;
     const long long ra = r0 + 2 * p, rb = ra + 1;
     const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
