@@ -180,7 +180,8 @@ def test_reference_potential_module(n, N, idx, in_golden_dir):
 
 
 # ------------------------------------------------------------------ step compositions
-@pytest.mark.parametrize("dim,N,L", [(2, 8, 1.0), (2, 32, 10.0), (3, 8, 1.0), (3, 16, 20.0)])
+@pytest.mark.parametrize("dim,N,L", [(2, 8, 1.0), (2, 32, 10.0), (3, 8, 1.0), (3, 16, 20.0),
+                                     (2, 512, 10.0), (3, 64, 20.0)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_fused_leapfrog_matches_oracle(qx, dim, N, L, prec):
     ds, ns = 2e-3, 7
