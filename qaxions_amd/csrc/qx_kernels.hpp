@@ -74,12 +74,14 @@ template <int N, typename R> struct Geom {
     // c2r_rows.  Where a line is one wavefront (TPL = 64) the kernel is
     // latency-limited and its registers allow 5 waves per SIMD: it takes
     // blocks of 4 waves with one exchange line per pair (22 KB of LDS), so
-    // that 5 blocks fit a CU.  Otherwise it keeps the r2c_rows blocks and
-    // transforms a pair's two psi rows in lockstep, each with its own
-    // exchange line, where two lines per pair fit in LDS beside the twiddles
-    // (else one after the other through one line).
+    // that 5 blocks fit a CU.  Lines of two wavefronts (N = 1024) also take
+    // 4-wave blocks, so that each exchange's block barrier waits for 4 waves
+    // instead of 8.  Otherwise it keeps the r2c_rows blocks and transforms a
+    // pair's two psi rows in lockstep, each with its own exchange line, where
+    // two lines per pair fit in LDS beside the twiddles (else one after the
+    // other through one line).
     static constexpr bool WAVE_LINE = PR::TPL == 64;
-    static constexpr int NPC = (WAVE_LINE && NP > 4) ? 4 : NP;
+    static constexpr int NPC = (WAVE_LINE && NP > 4) ? 4 : ((PR::TPL == 128 && NP > 2) ? 2 : NP);
     static constexpr int TLC = 2 * NPC;               // rows per block
     static constexpr int ROW_THREADS_C = NPC * PR::TPL;
     static constexpr bool TWO_LINES =
