@@ -197,6 +197,25 @@ def test_fused_leapfrog_matches_oracle(qx, dim, N, L, prec):
     assert lat.vmax == pytest.approx(np.abs(Vw).max(), rel=(1e-12 if prec == "fp64" else 1e-5))
 
 
+@pytest.mark.parametrize("dim,N,L,ns", [(2, 2, 1.0, 5), (2, 4, 1.0, 5), (3, 2, 1.0, 5), (3, 4, 1.0, 5),
+                                        (2, 2048, 40.0, 2), (2, 4096, 40.0, 2)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_leapfrog_extreme_sizes(qx, dim, N, L, ns, prec):
+    """The smallest lattices (single-stage transforms, fewer rows than a
+    row block) and the largest 2D ones, against the oracle."""
+    ds = 1e-3
+    lat = lattice(qx, dim, N, L, prec, pref=1.0)
+    psi = o.ic_grf(dim, N, L, 31) if N >= 16 else (
+        rng.standard_normal((N,) * dim) + 1j * rng.standard_normal((N,) * dim) + 2.0)
+    lat.psi = psi
+    lat.update_potential()
+    V, _, _ = o.update_potential(psi, L, 1.0)
+    lat.leapfrog(ds, ns)
+    want, Vw = o.leapfrog(psi, V, L, ds, ns, 1.0)
+    assert rel_l2(lat.psi, want) < (1e-12 if prec == "fp64" else 5e-6)
+    assert rel_l2(lat.V, Vw) < (1e-11 if prec == "fp64" else 5e-5)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_leapfrog_segments_and_propagate(qx, prec):
     dim, N, L, ds = 2, 64, 10.0, 1e-3
