@@ -220,7 +220,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64" if args.precision == "fp32" else "c128",
+        "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic: seeded Gaussian random field IC, P(k)~k^-2 cut at k_Ny/2",
         "config": {"workload": f"KDK Schroedinger-Poisson step, {dim}D {N}^{dim}, L={L}, dt={dt}, "
                                f"pref=1, rho_bar=1, k=0 zeroed, GRF seed 0",
@@ -316,7 +316,7 @@ def bench_ensemble(qx, args):
     B = sum(((9 * c + 5 * r) + 4 * c * (N // 2 + 1) / N) * N * N for N in Ns)
     out = {"metric": "ensemble site-updates/s (96 uneven 2D lattices, FP32)",
            "value": S * steps / (ms * 1e-3), "unit": "site-updates/s", "n_gpus": 1, "steps": steps,
-           "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "c64",
+           "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic: per-member GRF ICs, seeds 0-95",
            "config": {"workload": "32x128^2 + 32x512^2 + 24x1024^2 + 8x2048^2, L~U[5,40], "
                                   "dt=1e-3 (dx/dx0)^2, hipGraph 16 steps/replay",
@@ -347,7 +347,7 @@ def bench_learned(qx, args):
            "train_pairs": int(len(losses)), "train_seconds": t_train,
            "relative_loss_first_1k": float(relative[:1000].mean()),
            "relative_loss_last_1k": float(relative[-1000:].mean()),
-           "eval": res, "dtype": "c64 state, bf16 weights, fp32 accumulate",
+           "eval": res, "dtype": "f32 (bf16 network weights, fp32 accumulation)",
            "config": {"workload": f"2D {N}^2, L={L}, pref={pref}, fine dt={dt}, coarse {k} dt, "
                                   f"{args.eval_steps} coarse steps, mass tol 1e-6 per 10-step chunk"}}
     print(json.dumps(out), flush=True)
