@@ -141,8 +141,12 @@ struct qx_ctx {
     float2 *ftab = nullptr;
     double ptab_dt = std::nan("");
     // a scheduled leapfrog's per-step drift tables, SCHED_CHUNK steps at a
-    // time: step i's N double2 then N float2 at i * N * 24 bytes (lazy)
+    // time, double-buffered: step i's N double2 then N float2 at i * N * 24
+    // bytes of a buffer (lazy); pinned host staging of the same shape, and
+    // per buffer the event after its last upload
     unsigned char *sched = nullptr;
+    unsigned char *h_sched = nullptr;
+    hipEvent_t sched_up[2] = {nullptr, nullptr};
     // captured steady-state steps for small lattices (launch-bound)
     hipGraphExec_t gexec = nullptr;
     double g_ds = std::nan("");
@@ -609,6 +613,9 @@ qx_status qx_destroy(qx_ctx *c)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_partials) (void)hipHostFree(c->h_partials);
+    if (c->h_sched) (void)hipHostFree(c->h_sched);
+    for (hipEvent_t e : c->sched_up)
+        if (e) (void)hipEventDestroy(e);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1023,30 +1030,41 @@ qx_status qx_leapfrog_schedule(qx_ctx *c, int nsteps, const double *drift_dt, co
     }
     const int N = c->N;
     const size_t step_bytes = (sizeof(double2) + sizeof(float2)) * (size_t)N;
-    TRY(ensure(c, (void **)&c->sched, step_bytes * SCHED_CHUNK));
+    const size_t buf_bytes = step_bytes * SCHED_CHUNK;
+    TRY(ensure(c, (void **)&c->sched, 2 * buf_bytes));
+    if (!c->h_sched) HIPCHK(hipHostMalloc((void **)&c->h_sched, 2 * buf_bytes, hipHostMallocDefault));
+    for (hipEvent_t &e : c->sched_up)
+        if (!e) {
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(e, c->stream));  // a recorded state to wait on before the first upload
+        }
     const bool t = tiled(c);
     if (t) TRY(ensure(c, &c->psit, (size_t)c->S * c->csz));
     if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
     const MulArgs pm = poisson_args(c);
     const int np = step_pass_count(c);
     const long long rpm = c->dim == 3 ? N : c->rows;
-    std::vector<unsigned char> host(step_bytes * SCHED_CHUNK);
     std::vector<double2> T(N);
-    for (int j0 = 0; j0 < nsteps; j0 += SCHED_CHUNK) {
-        const int m = std::min(SCHED_CHUNK, nsteps - j0);
+    // chunk k's tables go through buffer k % 2: the host fills them while
+    // the device runs chunk k - 1, and the upload is ordered after chunk
+    // k - 2, the last reader of that device buffer
+    for (int k = 0, j0 = 0; j0 < nsteps; ++k, j0 += SCHED_CHUNK) {
+        const int m = std::min(SCHED_CHUNK, nsteps - j0), buf = k & 1;
+        unsigned char *hb = c->h_sched + buf * buf_bytes, *db = c->sched + buf * buf_bytes;
+        HIPCHK(hipEventSynchronize(c->sched_up[buf]));  // the upload from hb two chunks ago is done
         for (int i = 0; i < m; ++i) {
             fill_phase_table(T.data(), N, c->L, drift_dt[j0 + i]);
-            unsigned char *h = host.data() + step_bytes * i;
+            unsigned char *h = hb + step_bytes * i;
             std::memcpy(h, T.data(), sizeof(double2) * N);
             fill_phase_m1((float2 *)(h + sizeof(double2) * N), T.data(), N);
         }
-        HIPCHK(hipStreamSynchronize(c->stream));  // the previous chunk's passes are done with the tables
-        HIPCHK(hipMemcpy(c->sched, host.data(), step_bytes * m, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(db, hb, step_bytes * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->sched_up[buf], c->stream));
         for (int i = 0; i < m; ++i) {
             const int j = j0 + i;
             MulArgs am = axis_drift_args(c);
-            am.ptab = (const double2 *)(c->sched + step_bytes * i);
-            am.ftab = (const float2 *)(c->sched + step_bytes * i + sizeof(double2) * N);
+            am.ptab = (const double2 *)(db + step_bytes * i);
+            am.ftab = (const float2 *)(db + step_bytes * i + sizeof(double2) * N);
             if (j == 0)  // the opening kick reads the stored V
                 c->eng->c2r_rows(nullptr, c->V, c->psi, c->rows, rpm, kick_dt[0], nullptr, C2R_KICK | C2R_DRIFT,
                                  nullptr, c->stream, nullptr, am.ptab, am.ftab, 0, 1.0 / N);
