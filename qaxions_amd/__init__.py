@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Lattice", "Ensemble", "Corrector", "QxError", "lib", "LIB_PATH", "physics_static"]
+__all__ = ["Lattice", "Ensemble", "Snapshot", "Corrector", "QxError", "lib", "LIB_PATH", "physics_static"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqx.so")
