@@ -133,8 +133,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
-              "multi-GPU slab decomposition is a design only (see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU (a single MI355X); the multi-GPU slab decomposition "
+              "with all-to-all transposes is designed, not built.", file=sys.stderr)
         sys.exit(2)
 
     import qaxions_amd as qx

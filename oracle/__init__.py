@@ -1,7 +1,7 @@
 """FP64 CPU oracle for the KDK Schroedinger-Poisson path (TEST INFRASTRUCTURE).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` import this package, and only as the checker.  Product code
+``bench.py`` import this package, and only to compare against.  Product code
 (``qaxions_amd``) never imports it.
 
 Thin ctypes/numpy wrappers over ``liboqx.so`` built from ``qx_oracle.c``

@@ -1,5 +1,5 @@
 """FP64 numpy restatement of the learned coarse-step corrector (TEST
-INFRASTRUCTURE: the checker of the HIP forward/backward kernels).
+INFRASTRUCTURE: what the HIP forward/backward kernels are compared against).
 
 No reference counterpart.  The network: four periodic 3x3 convolutions
 3 -> 32 -> 32 -> 32 -> 2 with GELU (erf form) after the first three; the

@@ -1,6 +1,6 @@
 /*
  * qx_oracle.c -- FP64 CPU restatement of the qaxions KDK Schroedinger-Poisson
- * hot path.  TEST INFRASTRUCTURE ONLY (the checker): tests/, smoke() and the
+ * hot path.  TEST INFRASTRUCTURE ONLY (results are compared against it): tests/, smoke() and the
  * bench's cpu_baseline leg may call it, product code never does.
  *
  * Every function cites the reference file:line whose behaviour it restates.

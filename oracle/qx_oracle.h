@@ -1,8 +1,8 @@
 /*
  * qx_oracle.h -- FP64 CPU restatement of qaxions' kick-drift-kick
- * Schroedinger-Poisson step.  TEST INFRASTRUCTURE ONLY: the oracle is the
- * checker that tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg compare the HIP library against.  No product code links it.
+ * Schroedinger-Poisson step.  TEST INFRASTRUCTURE ONLY: the oracle is what
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg compare
+ * the HIP library against.  No product code links it.
  *
  * Conventions (all follow the reference):
  *   - lattices are N^dim, dim in {2,3}, C row-major, last index fastest;
@@ -53,7 +53,7 @@ double qxo_compute_energy(int dim, int N, const double *psi, double *V);
 void qxo_density_stats(int dim, int N, const double *psi, double *rhomax_over_mean,
                        double *sum_rho);
 
-/* ---- integrals (north-star additions, no reference counterpart) -------
+/* ---- integrals (additions with no reference counterpart) --------------
  * mass = dV sum|psi|^2;  kinetic = 1/2 dV/S sum_k k^2 |psihat_k|^2;
  * potential = 1/2 dV sum V |psi|^2, dV = (L/N)^dim. */
 void qxo_mass_energy(int dim, int N, double L, const double *psi, const double *V,

@@ -1,5 +1,5 @@
 // parse.h -- the reference's run parameters (src/parse.h:10-35) plus the
-// device fields this build adds.  defaults() sets the reference's values
+// device fields added here.  defaults() sets the reference's values
 // (src/parse.cxx:7-29).
 #ifndef QX_SHIM_PARSE_H
 #define QX_SHIM_PARSE_H

@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) with the FP64 oracle on the same
 seeded inputs.  Tolerances: FP64 contexts 1e-12 relative, FP32 contexts as
-stated per test (<= 1e-5 relative L2 after 100 steps)."""
+stated per test (the FP32 bar: <= 1e-5 relative L2 after 100 steps)."""
 import math
 
 import numpy as np
