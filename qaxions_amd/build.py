@@ -6,6 +6,7 @@
   libqxio.so        HDF5 configuration files (IO, SnapshotWriter) and measure(),
                     built when an HDF5 C library is found (HDF5_DIR, default /opt/conda)
   h5test*.so        pybind11 module with the reference's h5test entry points
+  qx_run            the reference's time-loop program over the device Field (C++)
 
 Run ``python -m qaxions_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -31,6 +32,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 LIB = os.path.join(HERE, "libqx.so")
 SHIM = os.path.join(HERE, "libqxshim.so")
 IOLIB = os.path.join(HERE, "libqxio.so")
+RUN = os.path.join(HERE, "qx_run")
 HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 
 
@@ -109,6 +111,11 @@ def build_io():
           os.path.join(CSRC, "shim", "h5test.cxx"),
           "-L", HERE, "-lqxio", "-lqxshim", "-lqx", "-Wl,-rpath,$ORIGIN", "-o", mod + ".tmp"])
     os.replace(mod + ".tmp", mod)
+    # the reference's program over the device Field, a C++ caller of the
+    # libraries (links the C++ runtime ahead of libqxio)
+    _run(["g++", "-O2", "-std=c++17", *inc, os.path.join(CSRC, "shim", "qx_run.cxx"),
+          "-L", HERE, "-lqxio", "-lqxshim", "-lqx", "-Wl,-rpath,$ORIGIN", "-o", RUN + ".tmp"])
+    os.replace(RUN + ".tmp", RUN)
     return IOLIB, mod
 
 

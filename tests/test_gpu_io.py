@@ -116,3 +116,39 @@ def test_snapshot_abi(qx, dim, N, prec):
         qx.Snapshot(lat, psi=False, V=False)
     s.close()
     v_only.close()
+
+
+def test_qx_run_program_follows_the_reference_loop(h5, in_golden_dir, tmp_path):
+    """The C++ program (the reference's src/main.cxx:58-85 over the device
+    Field, linked like a reference build) runs, writes one spectrum and one
+    grid file per measurement and the final state, and ends where the
+    oracle's restatement of the same loop ends, including the reference's
+    V <- |psi|^2 from computeEnergy before the kick after each measurement."""
+    import subprocess
+    import oracle as o
+    import refmath as rm
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qaxions_amd", "qx_run")
+    N, dim, nsteps, nmeas = 16, 3, 12, 3
+    print(f"[case] qx_run N={N} dim={dim} steps={nsteps}", flush=True)
+    r = subprocess.run([exe, str(N), str(dim), str(nsteps), str(nmeas), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    mlist = o.meas_list(nsteps, nmeas)                      # [0, 5, 11]
+    for m in range(len(mlist) + 1):
+        assert (tmp_path / f"spec_{m:04d}.txt").exists()
+        hdr, _ = h5.read_conf_arrays(str(tmp_path / f"field_{m:04d}.h5"), with_psi=False)
+        assert hdr["step"] == (0 if m == 0 else mlist[m - 1] + 1)
+    hdr, psi = h5.read_conf_arrays(str(tmp_path / "final.h5"))
+    assert hdr["step"] == nsteps
+    # the oracle: defaults() with norm 1 (STATIC: pref = 4 pi, rho_bar = 1, L = 1, dt = 1e-4)
+    L, dt, pref = 1.0, 1e-4, 4 * np.pi
+    p = o.ic_solitons(dim, N, L, rm.read_solitons())
+    V, _, _ = o.update_potential(p, L, pref)
+    p = o.kick(p, V, dt / 2)
+    for idx in range(nsteps):
+        p = o.drift(p, L, dt)
+        V, _, _ = o.update_potential(p, L, pref)
+        if idx in mlist:
+            V = np.abs(p) ** 2                               # computeEnergy in measure()
+        p = o.kick(p, V, dt if idx != nsteps - 1 else dt / 2)
+    assert rel_l2(psi.reshape(p.shape), p) < 1e-12
