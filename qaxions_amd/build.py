@@ -3,6 +3,7 @@
   libqx.so          HIP kernels + C ABI (include/qx.h), hipcc --offload-arch=gfx950
   libqxshim.so      C++ Field/FFTBackend/Spectrum mirror of the reference, over the C ABI
   fieldtest*.so     pybind11 module with the reference's fieldtest entry points
+  ictest*.so        pybind11 module with the reference's ictest entry points
   libqxio.so        HDF5 configuration files (IO, SnapshotWriter) and measure(),
                     built when an HDF5 C library is found (HDF5_DIR, default /opt/conda)
   h5test*.so        pybind11 module with the reference's h5test entry points
@@ -83,6 +84,12 @@ def build_shim():
           os.path.join(CSRC, "shim", "fieldtest.cxx"),
           "-L", HERE, "-lqxshim", "-lqx", "-Wl,-rpath,$ORIGIN", "-o", mod + ".tmp"])
     os.replace(mod + ".tmp", mod)
+    icmod = os.path.join(HERE, "ictest" + ext)
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc,
+          "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+          os.path.join(CSRC, "shim", "ictest.cxx"),
+          "-L", HERE, "-lqxshim", "-lqx", "-Wl,-rpath,$ORIGIN", "-o", icmod + ".tmp"])
+    os.replace(icmod + ".tmp", icmod)
     return SHIM, mod
 
 

@@ -36,6 +36,8 @@ def test_shim_and_pybind_module_load(qx):
     import qaxions_amd.fieldtest as ft
     for name in ("test_kick", "test_drift", "test_potential", "test_kick_drift_step"):
         assert callable(getattr(ft, name))
+    import qaxions_amd.ictest as ic
+    assert callable(ic.test_solitons) and callable(ic.test_solitons_3D)
 
 
 @pytest.mark.parametrize("dim,N,L,prec,msg", [

@@ -170,6 +170,19 @@ def test_reference_fieldtest_module(n, N, dt, idx, fp32, in_golden_dir):
             g[f"kds_{n}_{N}_{dt}_{idx}"], abs=1e-12)
 
 
+@pytest.mark.parametrize("N", [4, 8])
+@pytest.mark.parametrize("idx", [1, 9])
+def test_reference_ictest_module(N, idx, in_golden_dir):
+    """tests/test_ic.py:11-37 through the pybind11 module (the reference's
+    tolerance, and the FP64 host IC to 1e-14)."""
+    import qaxions_amd.ictest as ic
+    for dim, f in ((2, ic.test_solitons), (3, ic.test_solitons_3D)):
+        want = rm.solitons(dim, N).flat[idx].real
+        got = f(N, idx)
+        assert got == pytest.approx(want, abs=1e-5, rel=1e-4)
+        assert abs(got - want) < 1e-14
+
+
 @pytest.mark.parametrize("n,N,idx", rm.PARAMS_POT)
 def test_reference_potential_module(n, N, idx, in_golden_dir):
     import qaxions_amd.fieldtest as field
