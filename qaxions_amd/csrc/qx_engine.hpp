@@ -61,18 +61,18 @@ template <int N, typename R> struct EngineT final : Engine {
         else contig_fft<N, R, OP_POISSON><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
     }
 
-    // the tile width is the full column tile, whatever the variant of the
-    // plain passes: the tile layout must not depend on tuning
+    // writes tiles of width G::TT from blocks of G::TC columns: the tile
+    // layout does not depend on tuning
     void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
                              hipStream_t s) override
     {
-        constexpr int TC = G::TC, CPT = 1;
+        constexpr int TC = G::TC, CPT = 1, TT = G::TT;
         const long long ntiles = inner / TC;
         const unsigned blocks = (unsigned)(outer * ntiles);
-        strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
+        strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT, TT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
             (C *)data, inner, ntiles, m, (const C *)tw, (C *)tiles, outer);
     }
-    int tile_cols() const override { return G::TC; }
+    int tile_cols() const override { return G::TT; }
 
     template <int TC>
     void strided_tc(void *data, long long outer, long long inner, int op, const MulArgs &m, hipStream_t s)

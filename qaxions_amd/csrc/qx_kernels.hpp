@@ -50,6 +50,11 @@ template <int N, typename R> struct Geom {
     static constexpr int TC = (TC1 * P::TPL > 1024) ? (1024 / P::TPL) : TC1;
     static constexpr int STRIDED_THREADS = TC * P::TPL;
     static constexpr int TC_HALF = TC > 1 ? TC / 2 : 1;   // narrow-tile variant
+    // width of psi's column tiles (3D): the strided block width, except at
+    // N = 1024 in FP32, whose tiles are 16 columns wide so that the row
+    // kernels read whole 128 B tile rows; its strided passes (8 columns per
+    // block) then work on half tiles
+    static constexpr int TT = (N == 1024 && sizeof(C) == 8) ? 16 : TC;
     // row kernels: pairs of rows per block
     static constexpr int NP0 = (256 / PR::TPL) > 8 ? (256 / PR::TPL) : 8;
     static constexpr int NP1 = (NP0 * PR::TPL > 1024) ? (1024 / PR::TPL) : NP0;
@@ -362,7 +367,7 @@ enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 // CPT columns per thread: a block of TC/CPT thread columns transforms its
 // CPT columns one after another through one LDS area (CPT = 2 keeps a
 // 16-column tile's 128-byte row segments with 512 threads per block).
-template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1>  // OP_FWD, OP_INV, OP_DRIFT_AXIS or OP_POISSON
+template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1, int TT = TC>  // OP_FWD, OP_INV, OP_DRIFT_AXIS or OP_POISSON
 __global__ void __launch_bounds__(((TC / CPT) * Plan<N>::TPL))
 strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
             MulArgs mul, const typename cplx<R>::T *__restrict__ twg,
@@ -394,7 +399,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         for (int i = threadIdx.x; i < N; i += blockDim.x) ptab_lds[i] = ma.ptab[i];
         ptab = ptab_lds;
     }
-    C *const tile0 = tiles + ((b % ntiles) * outer + o) * (long long)N * TC + c;
+    C *const tile0 = tiles + ((col0 / TT) * outer + o) * (long long)N * TT + (col0 % TT) + c;
     C *const base0 = data + o * (long long)N * inner + col0 + c;
     C v[CPT][P::E];
 #pragma unroll
@@ -440,7 +445,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         if (col0 + c + j * TW < inner) {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
-                if constexpr (XM == XM_STORE_TILES) tile0[(t + q * P::TPL) * TC + j * TW] = v[j][q];
+                if constexpr (XM == XM_STORE_TILES) tile0[(t + q * P::TPL) * TT + j * TW] = v[j][q];
                 else base0[(long long)(t + q * P::TPL) * inner + j * TW] = v[j][q];
             }
         }
@@ -487,8 +492,8 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     } else {
         C a[P::E], b[P::E];
         if constexpr (MODE == R2C_DENSITY_TILES) {
-            load_tile_row<C, N, G::TC, P::TPL>(a, psi, ra, t, oka);
-            load_tile_row<C, N, G::TC, P::TPL>(b, psi, rb, t, okb);
+            load_tile_row<C, N, G::TT, P::TPL>(a, psi, ra, t, oka);
+            load_tile_row<C, N, G::TT, P::TPL>(b, psi, rb, t, okb);
         } else {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
@@ -584,8 +589,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     C a[P::E], b[P::E];
     if (upd) {
         if (flags & C2R_SRC_TILES) {
-            load_tile_row<C, N, G::TC, P::TPL>(a, tiles, ra, t, oka);
-            load_tile_row<C, N, G::TC, P::TPL>(b, tiles, rb, t, okb);
+            load_tile_row<C, N, G::TT, P::TPL>(a, tiles, ra, t, oka);
+            load_tile_row<C, N, G::TT, P::TPL>(b, tiles, rb, t, okb);
         } else {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
