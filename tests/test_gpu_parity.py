@@ -420,6 +420,29 @@ def test_headline_512cube_fp32_against_fp64_100_steps(qx):
     assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-5
 
 
+def test_1024cube_fp32_against_fp64(qx):
+    """3D 1024^3, the largest lattice: FP32 (16-column tiles, two-wavefront
+    lines) against FP64 (4-column tiles) from the same GRF IC, 4 steps."""
+    N, L, dt, steps = 1024, 40.0, 2.5e-4, 4
+    out, drift = {}, {}
+    for prec in ("fp64", "fp32"):
+        print(f"[case] 1024^3 {prec} {steps} steps", flush=True)
+        lat = qx.Lattice(3, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.ic_grf(0, 0.1)
+        lat.update_potential()
+        _, m0 = lat.density_stats()
+        lat.leapfrog(dt, steps)
+        _, m1 = lat.density_stats()
+        out[prec] = lat.psi
+        drift[prec] = abs(m1 - m0) / m0
+        lat.close()
+    err = rel_l2(out["fp32"], out["fp64"])
+    del out
+    print(f"1024^3 fp32 vs fp64 after {steps} steps: {err:.3e}; mass drift {drift}")
+    assert err <= 2e-6
+    assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-6
+
+
 # ------------------------------------------------------------------ initial conditions
 @pytest.mark.parametrize("dim,N", [(2, 64), (3, 16)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
