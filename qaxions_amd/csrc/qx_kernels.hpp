@@ -146,6 +146,29 @@ __device__ __forceinline__ MulArgs member_args(const MulArgs &a, long long membe
     return m;
 }
 
+// Streaming (non-temporal) stores of a pass's outputs: no pass re-reads
+// what it writes, and the next pass finds it in HBM anyway at the lattice
+// sizes that matter (each array is larger than L2 and the memory-side cache)
+typedef float vf2_t __attribute__((ext_vector_type(2)));
+typedef double vd2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(float2 *p, float2 v) { __builtin_nontemporal_store(vf2_t{v.x, v.y}, (vf2_t *)p); }
+__device__ __forceinline__ void st_nt(double2 *p, double2 v) { __builtin_nontemporal_store(vd2_t{v.x, v.y}, (vd2_t *)p); }
+__device__ __forceinline__ void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+// Streaming loads of inputs that a pass reads once, in whole 128 B lines
+// per block (an input line shared by two blocks keeps the plain load, so
+// that the second block can still find it in L2)
+__device__ __forceinline__ float2 ld_nt(const float2 *p)
+{
+    const vf2_t v = __builtin_nontemporal_load((const vf2_t *)p);
+    return float2{v.x, v.y};
+}
+__device__ __forceinline__ double2 ld_nt(const double2 *p)
+{
+    const vd2_t v = __builtin_nontemporal_load((const vd2_t *)p);
+    return double2{v.x, v.y};
+}
+
 // index of (row = i0*N + i1, i2) of a 3D lattice in its column tiles
 // [i2/TT][i0][i1][i2%TT] (TT columns per tile)
 template <int N, int TT>
@@ -165,10 +188,10 @@ __device__ __forceinline__ void load_tile_row(C (&v)[E], const C *__restrict__ t
         const C *p = tiles + tile_index<N, TT>(row, t);
         constexpr long long QS = (long long)(TPL / TT) * N * (long long)(N * TT);
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = ok ? p[q * QS] : C{0, 0};
+        for (int q = 0; q < E; ++q) v[q] = ok ? ld_nt(p + q * QS) : C{0, 0};
     } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = ok ? tiles[tile_index<N, TT>(row, t + q * TPL)] : C{0, 0};
+        for (int q = 0; q < E; ++q) v[q] = ok ? ld_nt(tiles + tile_index<N, TT>(row, t + q * TPL)) : C{0, 0};
     }
 }
 
@@ -321,7 +344,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
     C v[P::E];
     C *src = data + line * N;
 #pragma unroll
-    for (int q = 0; q < P::E; ++q) v[q] = ok ? src[t + q * P::TPL] : C{0, 0};
+    for (int q = 0; q < P::E; ++q) v[q] = ok ? ld_nt(src + t + q * P::TPL) : C{0, 0};
     __syncthreads();
     if constexpr (OP == OP_FWD) {
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
@@ -343,7 +366,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
     if (ok) {
         C *dst = out ? out + ((line % rm_m) * rm_a + line / rm_m) * N : src;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) dst[t + q * P::TPL] = v[q];
+        for (int q = 0; q < P::E; ++q) st_nt(dst + t + q * P::TPL, v[q]);
     }
 }
 
@@ -406,7 +429,11 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     for (int j = 0; j < CPT; ++j) {
         const bool ok = col0 + c + j * TW < inner;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) v[j][q] = ok ? base0[(long long)(t + q * P::TPL) * inner + j * TW] : C{0, 0};
+        for (int q = 0; q < P::E; ++q) {
+            const C *src = base0 + (long long)(t + q * P::TPL) * inner + j * TW;
+            if constexpr (TC * sizeof(C) >= 128) v[j][q] = ok ? ld_nt(src) : C{0, 0};
+            else v[j][q] = ok ? *src : C{0, 0};
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -445,8 +472,10 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         if (col0 + c + j * TW < inner) {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
-                if constexpr (XM == XM_STORE_TILES) tile0[(t + q * P::TPL) * TT + j * TW] = v[j][q];
-                else base0[(long long)(t + q * P::TPL) * inner + j * TW] = v[j][q];
+                C *dst = XM == XM_STORE_TILES ? tile0 + (t + q * P::TPL) * TT + j * TW
+                                              : base0 + (long long)(t + q * P::TPL) * inner + j * TW;
+                if constexpr (TC * sizeof(C) >= 128) st_nt(dst, v[j][q]);
+                else *dst = v[j][q];
             }
         }
     }
@@ -498,8 +527,8 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 const int pos = t + q * P::TPL;
-                a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-                b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+                a[q] = oka ? ld_nt(psi + ra * N + pos) : C{0, 0};
+                b[q] = okb ? ld_nt(psi + rb * N + pos) : C{0, 0};
             }
         }
         __syncthreads();
@@ -542,7 +571,7 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __syncthreads();
     for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
         const int k = i / TL, l = i % TL;
-        if (r0 + l < rows) vt[vt_index(r0 + l, k, rpm, H1)] = stg[k * G::STP + l];
+        if (r0 + l < rows) st_nt(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
     }
 }
 
@@ -595,8 +624,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 const int pos = t + q * P::TPL;
-                a[q] = oka ? psi[ra * N + pos] : C{0, 0};
-                b[q] = okb ? psi[rb * N + pos] : C{0, 0};
+                a[q] = oka ? ld_nt(psi + ra * N + pos) : C{0, 0};
+                b[q] = okb ? ld_nt(psi + rb * N + pos) : C{0, 0};
             }
         }
     }
@@ -646,8 +675,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int pos = t + q * P::TPL;
-            if (oka) V[ra * N + pos] = va[q];
-            if (okb) V[rb * N + pos] = vb[q];
+            if (oka) st_nt(V + ra * N + pos, va[q]);
+            if (okb) st_nt(V + rb * N + pos, vb[q]);
         }
     }
     if (flags & C2R_VMAX) {
@@ -718,8 +747,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int pos = t + q * P::TPL;
-            if (oka) psi[ra * N + pos] = a[q];
-            if (okb) psi[rb * N + pos] = b[q];
+            if (oka) st_nt(psi + ra * N + pos, a[q]);
+            if (okb) st_nt(psi + rb * N + pos, b[q]);
         }
     }
 }
