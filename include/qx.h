@@ -261,8 +261,8 @@ void *qx_net_stream(qx_net *net);
 const char *qx_net_last_error(void);
 
 /* Autotuner: times both tile variants (full / half column tile) of every
- * strided pass of the fused step on scratch copies and keeps the faster per
- * pass kind.  variants (nullable, 8 ints): the choice per [op kind][stride
+ * strided pass of the fused step on scratch copies, twice in alternation,
+ * and keeps the faster per pass kind.  variants (nullable, 8 ints): the choice per [op kind][stride
  * class], op kinds forward, inverse, drift, Poisson. */
 qx_status qx_tune(qx_ctx *ctx, double ds, int reps, int *variants);
 /* installs variants recorded by an earlier qx_tune (e.g. from a cache) */

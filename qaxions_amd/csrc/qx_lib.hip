@@ -1109,11 +1109,15 @@ qx_status qx_tune(qx_ctx *c, double ds, int reps, int *variants)
     CHECK_CTX(c);
     if (reps < 1) return fail(QX_EINVAL, "reps must be >= 1");
     const int np = step_pass_count(c);
-    double ms[2][8];
-    for (int v = 0; v < 2; ++v) {
-        for (auto &row : c->eng->strided_variant) row[0] = row[1] = v;
-        TRY(qx_time_step_passes(c, ds, reps, ms[v], nullptr, 8));
-    }
+    // two rounds with the variants alternating, the minimum per pass: a
+    // single round timed the first variant while the clocks still ramped up
+    double ms[2][8], t[8];
+    for (int round = 0; round < 2; ++round)
+        for (int v = 0; v < 2; ++v) {
+            for (auto &row : c->eng->strided_variant) row[0] = row[1] = v;
+            TRY(qx_time_step_passes(c, ds, reps, t, nullptr, 8));
+            for (int p = 0; p < np; ++p) ms[v][p] = round ? std::min(ms[v][p], t[p]) : t[p];
+        }
     for (auto &row : c->eng->strided_variant) row[0] = row[1] = 0;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured launches used other tiles
     c->gexec = nullptr;
