@@ -35,6 +35,9 @@ struct Engine {
                           const double2 *ptab = nullptr, const float2 *ftab = nullptr,
                           long long ptab_stride = 0, double inv_n = 0.0) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
+    // streaming loads and stores in the passes (lattices whose arrays are far
+    // larger than the memory-side cache; instantiated for N >= 512 only)
+    bool stream_io = false;
     // tile variant per strided pass kind (0 = full tile, 1 = half tile),
     // indexed by [op kind][stride class]; set by qx_tune
     int strided_variant[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -50,31 +53,51 @@ template <int N, typename R> struct EngineT final : Engine {
     using G = Geom<N, R>;
     using P = Plan<N>;
 
-    void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s, void *out,
-                long long rm_m, long long rm_a) override
+    // streaming instantiations exist for N >= 512 only: no smaller N has
+    // arrays large enough to gain from them
+    static constexpr bool NT_OK = N >= 512;
+
+    template <bool NT>
+    void contig_t(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s, void *out,
+                  long long rm_m, long long rm_a)
     {
         const unsigned blocks = (unsigned)((nlines + G::LPB - 1) / G::LPB);
         C *d = (C *)data, *o = (C *)out;
         const C *t = (const C *)tw;
-        if (op < 0) contig_fft<N, R, OP_FWD><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
-        else if (op > 0) contig_fft<N, R, OP_INV><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
-        else contig_fft<N, R, OP_POISSON><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
+        if (op < 0) contig_fft<N, R, OP_FWD, NT><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
+        else if (op > 0) contig_fft<N, R, OP_INV, NT><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
+        else contig_fft<N, R, OP_POISSON, NT><<<blocks, G::CONTIG_THREADS, 0, s>>>(d, nlines, m, t, o, rm_m, rm_a);
+    }
+    void contig(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s, void *out,
+                long long rm_m, long long rm_a) override
+    {
+        if constexpr (NT_OK)
+            if (stream_io) return contig_t<true>(data, nlines, op, m, s, out, rm_m, rm_a);
+        contig_t<false>(data, nlines, op, m, s, out, rm_m, rm_a);
     }
 
     // writes tiles of width G::TT from blocks of G::TC columns: the tile
     // layout does not depend on tuning
-    void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
-                             hipStream_t s) override
+    template <bool NT>
+    void strided_tiles_drift_t(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
+                               hipStream_t s)
     {
         constexpr int TC = G::TC, CPT = 1, TT = G::TT;
         const long long ntiles = inner / TC;
         const unsigned blocks = (unsigned)(outer * ntiles);
-        strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT, TT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
+        strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT, TT, NT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
             (C *)data, inner, ntiles, m, (const C *)tw, (C *)tiles, outer);
+    }
+    void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
+                             hipStream_t s) override
+    {
+        if constexpr (NT_OK)
+            if (stream_io) return strided_tiles_drift_t<true>(data, tiles, outer, inner, m, s);
+        strided_tiles_drift_t<false>(data, tiles, outer, inner, m, s);
     }
     int tile_cols() const override { return G::TT; }
 
-    template <int TC>
+    template <int TC, bool NT>
     void strided_tc(void *data, long long outer, long long inner, int op, const MulArgs &m, hipStream_t s)
     {
         const long long ntiles = (inner + TC - 1) / TC;
@@ -82,20 +105,28 @@ template <int N, typename R> struct EngineT final : Engine {
         const int threads = TC * P::TPL;
         C *d = (C *)data;
         const C *t = (const C *)tw;
-        if (op < 0) strided_fft<N, R, OP_FWD, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
-        else if (op > 0) strided_fft<N, R, OP_INV, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        if (op < 0) strided_fft<N, R, OP_FWD, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        else if (op > 0) strided_fft<N, R, OP_INV, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
         else if (m.kind == MUL_DRIFT)
-            strided_fft<N, R, OP_DRIFT_AXIS, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
-        else strided_fft<N, R, OP_POISSON, TC><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+            strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        else strided_fft<N, R, OP_POISSON, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+    }
+    template <bool NT>
+    void strided_t(void *data, long long outer, long long inner, int op, const MulArgs &m, hipStream_t s)
+    {
+        if (variant(op, m, inner)) strided_tc<G::TC_HALF, NT>(data, outer, inner, op, m, s);
+        else strided_tc<G::TC, NT>(data, outer, inner, op, m, s);
     }
     void strided(void *data, long long outer, long long inner, int op, const MulArgs &m,
                  hipStream_t s) override
     {
-        if (variant(op, m, inner)) strided_tc<G::TC_HALF>(data, outer, inner, op, m, s);
-        else strided_tc<G::TC>(data, outer, inner, op, m, s);
+        if constexpr (NT_OK)
+            if (stream_io) return strided_t<true>(data, outer, inner, op, m, s);
+        strided_t<false>(data, outer, inner, op, m, s);
     }
-    void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
-                  double pref, double rho_bar, hipStream_t s) override
+    template <bool NT>
+    void r2c_rows_t(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
+                    double pref, double rho_bar, hipStream_t s)
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         C *p = (C *)psi;
@@ -103,11 +134,18 @@ template <int N, typename R> struct EngineT final : Engine {
         C *o = (C *)vt;
         const C *t = (const C *)tw;
         if (mode == R2C_REAL)
-            qx::r2c_rows<N, R, R2C_REAL><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_REAL, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else if (mode == R2C_DENSITY)
-            qx::r2c_rows<N, R, R2C_DENSITY><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_DENSITY, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
         else
-            qx::r2c_rows<N, R, R2C_DENSITY_TILES><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_DENSITY_TILES, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+    }
+    void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
+                  double pref, double rho_bar, hipStream_t s) override
+    {
+        if constexpr (NT_OK)
+            if (stream_io) return r2c_rows_t<true>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
+        r2c_rows_t<false>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
                   const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s,
@@ -115,10 +153,12 @@ template <int N, typename R> struct EngineT final : Engine {
                   double inv_n) override
     {
         const unsigned blocks = (unsigned)((rows + G::TLC - 1) / G::TLC);
-        qx::c2r_rows<N, R><<<blocks, G::ROW_THREADS_C, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm,
-                                                             (R)kick_dt, kick_member, flags, vmax,
-                                                             (const C *)tw, (const C *)tiles, ptab, ftab,
-                                                             ptab_stride, inv_n);
+        auto k = qx::c2r_rows<N, R, false>;
+        if constexpr (NT_OK)
+            if (stream_io) k = qx::c2r_rows<N, R, true>;
+        k<<<blocks, G::ROW_THREADS_C, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm, (R)kick_dt, kick_member,
+                                             flags, vmax, (const C *)tw, (const C *)tiles, ptab, ftab, ptab_stride,
+                                             inv_n);
     }
 };
 

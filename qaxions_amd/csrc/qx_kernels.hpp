@@ -169,6 +169,20 @@ __device__ __forceinline__ double2 ld_nt(const double2 *p)
     return double2{v.x, v.y};
 }
 
+// a pass's loads and stores: streaming (NT) or plain.  Streaming pays where
+// the lattice's arrays are far larger than the memory-side cache; smaller
+// lattices find the next pass's input there and keep plain accesses.
+template <bool NT, typename T> __device__ __forceinline__ void st(T *p, T v)
+{
+    if constexpr (NT) st_nt(p, v);
+    else *p = v;
+}
+template <bool NT, typename T> __device__ __forceinline__ T ld(const T *p)
+{
+    if constexpr (NT) return ld_nt(p);
+    else return *p;
+}
+
 // index of (row = i0*N + i1, i2) of a 3D lattice in its column tiles
 // [i2/TT][i0][i1][i2%TT] (TT columns per tile)
 template <int N, int TT>
@@ -181,17 +195,17 @@ __device__ __forceinline__ long long tile_index(long long row, int i2)
 // A thread's values of one row in the tiles, positions t + q*TPL: when TPL is
 // a multiple of TT they are at tile_index(row, t) + q * TILE_Q, a compile-time
 // stride, so the 64-bit index is formed once per row
-template <typename C, int N, int TT, int TPL, int E>
+template <typename C, int N, int TT, int TPL, bool NT, int E>
 __device__ __forceinline__ void load_tile_row(C (&v)[E], const C *__restrict__ tiles, long long row, int t, bool ok)
 {
     if constexpr (TPL % TT == 0) {
         const C *p = tiles + tile_index<N, TT>(row, t);
         constexpr long long QS = (long long)(TPL / TT) * N * (long long)(N * TT);
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = ok ? ld_nt(p + q * QS) : C{0, 0};
+        for (int q = 0; q < E; ++q) v[q] = ok ? ld<NT>(p + q * QS) : C{0, 0};
     } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = ok ? ld_nt(tiles + tile_index<N, TT>(row, t + q * TPL)) : C{0, 0};
+        for (int q = 0; q < E; ++q) v[q] = ok ? ld<NT>(tiles + tile_index<N, TT>(row, t + q * TPL)) : C{0, 0};
     }
 }
 
@@ -325,7 +339,7 @@ __device__ __forceinline__ void load_tw(C *tw, const C *twg)
 // two outer indices of a 3-index array swap places (rm_a = count of a), so a
 // block's LPB lines land in LPB whole lines of the other layout
 // ---------------------------------------------------------------------------
-template <int N, typename R, int OP>  // OP_FWD, OP_INV or OP_POISSON (2D vt lines)
+template <int N, typename R, int OP, bool NT>  // OP_FWD, OP_INV or OP_POISSON (2D vt lines)
 __global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
 contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul,
            const typename cplx<R>::T *__restrict__ twg, typename cplx<R>::T *__restrict__ out,
@@ -344,7 +358,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
     C v[P::E];
     C *src = data + line * N;
 #pragma unroll
-    for (int q = 0; q < P::E; ++q) v[q] = ok ? ld_nt(src + t + q * P::TPL) : C{0, 0};
+    for (int q = 0; q < P::E; ++q) v[q] = ok ? ld<NT>(src + t + q * P::TPL) : C{0, 0};
     __syncthreads();
     if constexpr (OP == OP_FWD) {
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
@@ -366,7 +380,7 @@ contig_fft(typename cplx<R>::T *__restrict__ data, long long nlines, MulArgs mul
     if (ok) {
         C *dst = out ? out + ((line % rm_m) * rm_a + line / rm_m) * N : src;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) st_nt(dst + t + q * P::TPL, v[q]);
+        for (int q = 0; q < P::E; ++q) st<NT>(dst + t + q * P::TPL, v[q]);
     }
 }
 
@@ -390,7 +404,9 @@ enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 // CPT columns per thread: a block of TC/CPT thread columns transforms its
 // CPT columns one after another through one LDS area (CPT = 2 keeps a
 // 16-column tile's 128-byte row segments with 512 threads per block).
-template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1, int TT = TC>  // OP_FWD, OP_INV, OP_DRIFT_AXIS or OP_POISSON
+// NT: streaming accesses where a block row is a whole 128 B line (narrower
+// rows share lines between blocks, which then merge in L2)
+template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1, int TT = TC, bool NT = false>  // OP_FWD, OP_INV, OP_DRIFT_AXIS or OP_POISSON
 __global__ void __launch_bounds__(((TC / CPT) * Plan<N>::TPL))
 strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
             MulArgs mul, const typename cplx<R>::T *__restrict__ twg,
@@ -431,8 +447,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const C *src = base0 + (long long)(t + q * P::TPL) * inner + j * TW;
-            if constexpr (TC * sizeof(C) >= 128) v[j][q] = ok ? ld_nt(src) : C{0, 0};
-            else v[j][q] = ok ? *src : C{0, 0};
+            v[j][q] = ok ? ld<NT && TC * sizeof(C) >= 128>(src) : C{0, 0};
         }
     }
     __syncthreads();
@@ -474,8 +489,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
             for (int q = 0; q < P::E; ++q) {
                 C *dst = XM == XM_STORE_TILES ? tile0 + (t + q * P::TPL) * TT + j * TW
                                               : base0 + (long long)(t + q * P::TPL) * inner + j * TW;
-                if constexpr (TC * sizeof(C) >= 128) st_nt(dst, v[j][q]);
-                else *dst = v[j][q];
+                st<NT && TC * sizeof(C) >= 128>(dst, v[j][q]);
             }
         }
     }
@@ -491,7 +505,7 @@ enum {
                             // tiles (the `psi` argument; see tile_index)
 };
 
-template <int N, typename R, int MODE>
+template <int N, typename R, int MODE, bool NT>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
          typename cplx<R>::T *__restrict__ vt, long long rows, long long rpm, R pref, R rho_bar,
@@ -521,14 +535,14 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     } else {
         C a[P::E], b[P::E];
         if constexpr (MODE == R2C_DENSITY_TILES) {
-            load_tile_row<C, N, G::TT, P::TPL>(a, psi, ra, t, oka);
-            load_tile_row<C, N, G::TT, P::TPL>(b, psi, rb, t, okb);
+            load_tile_row<C, N, G::TT, P::TPL, NT>(a, psi, ra, t, oka);
+            load_tile_row<C, N, G::TT, P::TPL, NT>(b, psi, rb, t, okb);
         } else {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 const int pos = t + q * P::TPL;
-                a[q] = oka ? ld_nt(psi + ra * N + pos) : C{0, 0};
-                b[q] = okb ? ld_nt(psi + rb * N + pos) : C{0, 0};
+                a[q] = oka ? ld<NT>(psi + ra * N + pos) : C{0, 0};
+                b[q] = okb ? ld<NT>(psi + rb * N + pos) : C{0, 0};
             }
         }
         __syncthreads();
@@ -571,7 +585,8 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __syncthreads();
     for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
         const int k = i / TL, l = i % TL;
-        if (r0 + l < rows) st_nt(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
+        // streamed only where a block's segment of a k-row is a whole 128 B line
+        if (r0 + l < rows) st<NT && G::TL * sizeof(C) >= 128>(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
     }
 }
 
@@ -589,7 +604,7 @@ enum {
                         // see tile_index); psi rows are written
 };
 
-template <int N, typename R>
+template <int N, typename R, bool NT>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS_C))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
@@ -618,14 +633,14 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     C a[P::E], b[P::E];
     if (upd) {
         if (flags & C2R_SRC_TILES) {
-            load_tile_row<C, N, G::TT, P::TPL>(a, tiles, ra, t, oka);
-            load_tile_row<C, N, G::TT, P::TPL>(b, tiles, rb, t, okb);
+            load_tile_row<C, N, G::TT, P::TPL, NT>(a, tiles, ra, t, oka);
+            load_tile_row<C, N, G::TT, P::TPL, NT>(b, tiles, rb, t, okb);
         } else {
 #pragma unroll
             for (int q = 0; q < P::E; ++q) {
                 const int pos = t + q * P::TPL;
-                a[q] = oka ? ld_nt(psi + ra * N + pos) : C{0, 0};
-                b[q] = okb ? ld_nt(psi + rb * N + pos) : C{0, 0};
+                a[q] = oka ? ld<NT>(psi + ra * N + pos) : C{0, 0};
+                b[q] = okb ? ld<NT>(psi + rb * N + pos) : C{0, 0};
             }
         }
     }
@@ -675,8 +690,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int pos = t + q * P::TPL;
-            if (oka) st_nt(V + ra * N + pos, va[q]);
-            if (okb) st_nt(V + rb * N + pos, vb[q]);
+            if (oka) st<NT>(V + ra * N + pos, va[q]);
+            if (okb) st<NT>(V + rb * N + pos, vb[q]);
         }
     }
     if (flags & C2R_VMAX) {
@@ -747,8 +762,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
 #pragma unroll
         for (int q = 0; q < P::E; ++q) {
             const int pos = t + q * P::TPL;
-            if (oka) st_nt(psi + ra * N + pos, a[q]);
-            if (okb) st_nt(psi + rb * N + pos, b[q]);
+            if (oka) st<NT>(psi + ra * N + pos, a[q]);
+            if (okb) st<NT>(psi + rb * N + pos, b[q]);
         }
     }
 }

@@ -158,6 +158,7 @@ namespace {
 
 constexpr long long STAGE_ELEMS = 1ll << 22;  // complex128 elements per transfer chunk
 constexpr int MAX_PART = 1024;
+constexpr long long STREAM_IO_MIN_BYTES = 512ll << 20;
 
 qx_status ensure(qx_ctx *c, void **p, size_t bytes)
 {
@@ -573,6 +574,10 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
     c->rsz = prec == QX_FP64 ? 8 : 4;
     c->eng = make_engine(N, prec == QX_FP64);
     if (!c->eng) return fail(QX_EINVAL, "unsupported N");
+    // streaming pass accesses for lattices whose psi is at least twice the
+    // 256 MB memory-side cache (3D 512^3 and up): measured faster there,
+    // slower where the arrays fit the cache (3D 256^3, 2D 4096^2)
+    c->eng->stream_io = (long long)c->S * (long long)c->csz >= STREAM_IO_MIN_BYTES;
 
     auto cleanup = [&](qx_status s) {
         qx_destroy(c.release());
