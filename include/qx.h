@@ -172,10 +172,11 @@ qx_status qx_ic_blobs(qx_ctx *ctx, const double *table, int nblobs, double bkg);
 /* passes of one fused steady-state KDK step (7 in 3D, 4 in 2D) */
 int qx_step_pass_count(const qx_ctx *ctx);
 const char *qx_step_pass_name(const qx_ctx *ctx, int pass);
-/* Times each pass alone, `reps` launches back to back on scratch copies of
- * the state (the state itself is left untouched), with HIP events on the
- * context's stream.  ms[pass] = average ms per launch; bytes[pass]
- * (nullable) = minimum HBM bytes the pass moves.  cap >= pass count. */
+/* Times each pass inside `reps` consecutive fused steps (after one warm-up
+ * step) on scratch copies of the state (the state itself is left
+ * untouched), with HIP events between the passes on the context's stream.
+ * ms[pass] = average ms per launch; bytes[pass] (nullable) = minimum HBM
+ * bytes the pass moves.  cap >= pass count. */
 qx_status qx_time_step_passes(qx_ctx *ctx, double ds, int reps, double *ms, double *bytes, int cap);
 
 /* ---- ensembles of independent 2D lattices (no reference counterpart: the

@@ -1175,23 +1175,34 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     if (c->dim == 3) TRY(ensure(c, &c->vtk, (size_t)c->H * c->csz));
     // pass 0 of a tiled lattice reads psi from its tiles: fill them first
     if (tiled(c)) step_pass(c, 1, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    // the passes inside consecutive steps, as the fused loop runs them: an
+    // event before every pass and one after the last, for one warm-up step
+    // and `reps` timed ones, read after a single synchronisation
+    const int steps = reps + 1, nev = steps * np + 1;
+    std::vector<hipEvent_t> ev(nev, nullptr);
     qx_status st = QX_OK;
+    for (int i = 0; i < nev && st == QX_OK; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) st = fail(QX_EDEVICE, "event creation failed");
+    for (int k = 0; k < steps && st == QX_OK; ++k)
+        for (int p = 0; p < np && st == QX_OK; ++p) {
+            if (hipEventRecord(ev[k * np + p], c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
+            step_pass(c, p, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
+        }
+    if (st == QX_OK && hipEventRecord(ev[nev - 1], c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
+    if (st == QX_OK && hipEventSynchronize(ev[nev - 1]) != hipSuccess) st = fail(QX_EDEVICE, "event timing failed");
     for (int p = 0; p < np && st == QX_OK; ++p) {
-        step_pass(c, p, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);  // warm
-        if (hipEventRecord(e0, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
-        for (int r = 0; r < reps; ++r) step_pass(c, p, ds, c->ph.pref, c->aux_c, c->psit, c->aux_vt, c->vtk, am, pm);
-        if (hipEventRecord(e1, c->stream) != hipSuccess) st = fail(QX_EDEVICE, "event record");
-        float t = 0.f;
-        if (st == QX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
-            st = fail(QX_EDEVICE, "event timing failed");
-        ms[p] = (double)t / reps;
+        double sum = 0.0;
+        for (int k = 1; k < steps && st == QX_OK; ++k) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[k * np + p], ev[k * np + p + 1]) != hipSuccess)
+                st = fail(QX_EDEVICE, "event timing failed");
+            sum += t;
+        }
+        ms[p] = sum / reps;
         if (bytes) bytes[p] = pass_bytes(c, p);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     if (st != QX_OK) return st;
     return sync(c);
 }

@@ -511,6 +511,31 @@ def test_ensemble_members_match_solo_runs(qx, prec, graph_steps):
     assert rel_l2(ens.psi(1), want) < (1e-12 if prec == "fp64" else 5e-6)
 
 
+def test_pass_timing_and_tune_keep_state(qx):
+    """Per-pass timing (inside consecutive steps on scratch copies) and the
+    tile-variant tuner leave the state untouched, time every pass, and the
+    chosen variants do not change results."""
+    for dim, N in ((3, 32), (2, 64)):
+        print(f"[case] pass timing dim={dim} N={N}", flush=True)
+        a = qx.Lattice(dim, N, 10.0, "fp32", pref=1.0)
+        b = qx.Lattice(dim, N, 10.0, "fp32", pref=1.0)
+        for lat in (a, b):
+            lat.ic_grf(5, 0.1)
+            lat.update_potential()
+        psi0, V0 = a.psi, a.V
+        names, ms, by = a.time_step_passes(1e-3, reps=3)
+        assert len(names) == (7 if dim == 3 else 4) and len(set(names)) == len(names)
+        assert np.all(ms > 0) and np.all(by > 0)
+        v = a.tune(1e-3, reps=2)
+        assert len(v) == 8 and set(v) <= {0, 1}
+        assert np.array_equal(a.psi, psi0) and np.array_equal(a.V, V0)
+        a.leapfrog(1e-3, 4)
+        b.leapfrog(1e-3, 4)
+        assert np.array_equal(a.psi, b.psi)
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_ensemble_tune_keeps_state_and_results(qx, prec):
     """The ensemble autotuner reports one tile variant per size class, leaves
