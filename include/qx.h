@@ -268,6 +268,12 @@ const char *qx_net_last_error(void);
 qx_status qx_tune(qx_ctx *ctx, double ds, int reps, int *variants);
 /* installs variants recorded by an earlier qx_tune (e.g. from a cache) */
 qx_status qx_set_tile_variants(qx_ctx *ctx, const int *variants);
+/* Streaming (non-temporal) loads and stores in the step's passes (no
+ * reference counterpart: a cache policy, results are bitwise the same).
+ * qx_create turns them on for lattices whose psi is at least 512 MB; on = 1
+ * or 0 overrides that.  Lattices with N < 512 have no streaming kernels and
+ * report 0.  *on_out (nullable) receives the setting in force. */
+qx_status qx_set_stream_io(qx_ctx *ctx, int on, int *on_out);
 
 /* ---- plumbing (no reference counterpart) -------------------------------- */
 void *qx_stream(qx_ctx *ctx);          /* the context's hipStream_t */

@@ -129,6 +129,7 @@ SIGNATURES = {
     "qx_net_last_error": (C.c_char_p, []),
     "qx_tune": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int)]),
     "qx_set_tile_variants": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "qx_set_stream_io": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "qx_stream": (C.c_void_p, [C.c_void_p]),
     "qx_synchronize": (C.c_int, [C.c_void_p]),
     "qx_last_error": (C.c_char_p, []),
@@ -394,6 +395,13 @@ class Lattice:
 
     def set_tile_variants(self, v):
         _check(lib().qx_set_tile_variants(self._h, (C.c_int * 8)(*v)))
+
+    def set_stream_io(self, on):
+        """Streaming (non-temporal) loads and stores in the step's passes;
+        returns the setting in force (False where N < 512)."""
+        out = C.c_int()
+        _check(lib().qx_set_stream_io(self._h, 1 if on else 0, C.byref(out)))
+        return bool(out.value)
 
     # -- plumbing ---------------------------------------------------------------------------
     @property

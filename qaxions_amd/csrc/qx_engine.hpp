@@ -38,6 +38,7 @@ struct Engine {
     // streaming loads and stores in the passes (lattices whose arrays are far
     // larger than the memory-side cache; instantiated for N >= 512 only)
     bool stream_io = false;
+    virtual bool has_stream_io() const = 0;
     // tile variant per strided pass kind (0 = full tile, 1 = half tile),
     // indexed by [op kind][stride class]; set by qx_tune
     int strided_variant[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -56,6 +57,7 @@ template <int N, typename R> struct EngineT final : Engine {
     // streaming instantiations exist for N >= 512 only: no smaller N has
     // arrays large enough to gain from them
     static constexpr bool NT_OK = N >= 512;
+    bool has_stream_io() const override { return NT_OK; }
 
     template <bool NT>
     void contig_t(void *data, long long nlines, int op, const MulArgs &m, hipStream_t s, void *out,

@@ -577,7 +577,7 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
     // streaming pass accesses for lattices whose psi is at least twice the
     // 256 MB memory-side cache (3D 512^3 and up): measured faster there,
     // slower where the arrays fit the cache (3D 256^3, 2D 4096^2)
-    c->eng->stream_io = (long long)c->S * (long long)c->csz >= STREAM_IO_MIN_BYTES;
+    c->eng->stream_io = c->eng->has_stream_io() && (long long)c->S * (long long)c->csz >= STREAM_IO_MIN_BYTES;
 
     auto cleanup = [&](qx_status s) {
         qx_destroy(c.release());
@@ -1133,6 +1133,17 @@ qx_status qx_tune(qx_ctx *c, double ds, int reps, int *variants)
     if (variants)
         for (int k = 0; k < 4; ++k)
             for (int cl = 0; cl < 2; ++cl) variants[2 * k + cl] = c->eng->strided_variant[k][cl];
+    return QX_OK;
+}
+
+qx_status qx_set_stream_io(qx_ctx *c, int on, int *on_out)
+{
+    if (!valid_ctx(c)) return fail(QX_EINVAL, "null context");
+    if (on != 0 && on != 1) return fail(QX_EINVAL, "on is 0 or 1");
+    c->eng->stream_io = on && c->eng->has_stream_io();
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured launches used the other kernels
+    c->gexec = nullptr;
+    if (on_out) *on_out = c->eng->stream_io ? 1 : 0;
     return QX_OK;
 }
 

@@ -420,6 +420,29 @@ def test_headline_512cube_fp32_against_fp64_100_steps(qx):
     assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-5
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_streaming_accesses_bitwise(qx, prec):
+    """Streaming (non-temporal) loads and stores change the cache policy only:
+    3 fused steps at 512^3 with them and without give the same bits.  They are
+    on by default at this size, and absent below N = 512."""
+    N, dt = 512, 2.5e-4
+    out = []
+    for on in (True, False):
+        print(f"[case] 512^3 {prec} streaming={on}", flush=True)
+        lat = qx.Lattice(3, N, 40.0, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        assert lat.set_stream_io(True) is True  # the default here
+        assert lat.set_stream_io(on) is on
+        lat.ic_grf(0, 0.1)
+        lat.update_potential()
+        lat.leapfrog(dt, 3)
+        out.append((lat.psi, lat.V))
+        lat.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    small = qx.Lattice(3, 64, 20.0, prec)
+    assert small.set_stream_io(True) is False
+    small.close()
+
+
 def test_1024cube_fp32_against_fp64(qx):
     """3D 1024^3, the largest lattice: FP32 (16-column tiles, two-wavefront
     lines) against FP64 (4-column tiles) from the same GRF IC, 4 steps."""
