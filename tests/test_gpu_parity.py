@@ -282,11 +282,13 @@ def test_mre_schedule_matches_oracle(qx, dim, N, L, steps, prec, tol):
     assert lat.vmax == pytest.approx(np.abs(Vw).max(), rel=10 * tol)
 
 
-@pytest.mark.parametrize("dim,N,prec", [(2, 64, "fp64"), (3, 32, "fp32"), (3, 8, "fp32")])
-def test_constant_schedule_is_leapfrog_bitwise(qx, dim, N, prec):
+@pytest.mark.parametrize("dim,N,prec,ns", [(2, 64, "fp64", 170), (3, 32, "fp32", 100), (3, 8, "fp32", 20)])
+def test_constant_schedule_is_leapfrog_bitwise(qx, dim, N, prec, ns):
     """Constant per-step values run the same kernels with the same tables as
-    qx_leapfrog (graph-captured there for these sizes)."""
-    L, ds, ns = 10.0, 1e-3, 20
+    qx_leapfrog (graph-captured there for these sizes).  170 steps take six
+    chunks of uploaded tables through both buffers; a second run continues
+    from the first, and a third runs with new physics."""
+    L, ds = 10.0, 1e-3
     psi = o.ic_grf(dim, N, L, 22)
     a = lattice(qx, dim, N, L, prec, pref=1.0)
     a.psi = psi
@@ -294,10 +296,15 @@ def test_constant_schedule_is_leapfrog_bitwise(qx, dim, N, prec):
     b = lattice(qx, dim, N, L, prec, pref=1.0)
     b.psi = psi
     b.update_potential()
-    a.leapfrog(ds, ns)
-    b.leapfrog_schedule([ds] * ns, [ds / 2] + [ds] * (ns - 1) + [ds / 2], [1.0] * ns)
-    assert np.array_equal(a.psi, b.psi)
-    assert np.array_equal(a.V, b.V)
+    for run in range(3):
+        if run == 2:
+            for lat in (a, b):
+                lat.set_physics(2.0, rho_bar=1.0, k0_mode=0)
+                lat.update_potential()
+        a.leapfrog(ds, ns)
+        b.leapfrog_schedule([ds] * ns, [ds / 2] + [ds] * (ns - 1) + [ds / 2], [a.get_physics()["pref"]] * ns)
+        assert np.array_equal(a.psi, b.psi), run
+        assert np.array_equal(a.V, b.V), run
     with pytest.raises(qx.QxError):
         b.leapfrog_schedule([], [ds], [])
     with pytest.raises(qx.QxError):
