@@ -255,6 +255,55 @@ def test_mass_conserved_energy_bounded():
     assert abs((k1 + p1) - (k0 + p0)) / abs(k0 + p0) < 1e-4
 
 
+def _two_level_density(dim, N, lo, hi, seed):
+    """psi with |psi|^2 = lo on the first half of axis 0 and hi on the second,
+    with random phases (both real and imaginary parts nonzero) and the two
+    halves shuffled, so a formula that drops Im psi, squares the wrong part or
+    depends on site order fails."""
+    r = np.random.default_rng(seed)
+    rho = np.full((N,) * dim, float(lo))
+    rho[N // 2:] = hi
+    rho = r.permutation(rho.reshape(-1)).reshape(rho.shape)
+    theta = r.uniform(0.1, 2 * np.pi - 0.1, rho.shape)
+    return np.sqrt(rho) * np.exp(1j * theta), rho
+
+
+@pytest.mark.parametrize("dim,N", [(2, 8), (2, 16), (3, 8)])
+def test_compute_energy_closed_form(dim, N):
+    """computeEnergy (src/field.cxx:200-227): V <- |psi|^2 and max/mean.  With
+    |psi|^2 in {1, 3} on two halves, max/mean = 3/2 and V is the two levels."""
+    psi, rho = _two_level_density(dim, N, 1.0, 3.0, 5)
+    V, r = o.compute_energy(psi)
+    assert np.abs(V - rho).max() < 1e-14
+    assert r == pytest.approx(1.5, rel=1e-14)
+    # a single spike of height A on a unit background: max/mean = A S/(S-1+A)
+    S = N ** dim
+    spike = np.ones((N,) * dim, complex) * np.exp(0.7j)
+    spike.reshape(-1)[S // 3] = math.sqrt(9.0) * np.exp(-1.1j)
+    V, r = o.compute_energy(spike)
+    assert r == pytest.approx(9.0 * S / (S - 1 + 9.0), rel=1e-14)
+    assert V.reshape(-1)[S // 3] == pytest.approx(9.0, rel=1e-15)
+
+
+@pytest.mark.parametrize("dim,N", [(2, 8), (3, 8)])
+def test_density_stats_closed_form(dim, N):
+    """The same statistics without touching V: max/mean = hi/((lo+hi)/2) and
+    sum |psi|^2 = S (lo + hi)/2 (mass = (L/N)^dim times that)."""
+    S = N ** dim
+    for lo, hi in [(1.0, 3.0), (0.1, 10.0), (2.0, 2.0)]:
+        psi, _ = _two_level_density(dim, N, lo, hi, 7)
+        r, s = o.density_stats(psi)
+        assert r == pytest.approx(hi / ((lo + hi) / 2), rel=1e-14)
+        assert s == pytest.approx(S * (lo + hi) / 2, rel=1e-14)
+    # agrees with computeEnergy's max/mean on any state, and mass_energy's mass
+    psi = o.ic_grf(dim, N, 6.0, 3)
+    r, s = o.density_stats(psi)
+    _, r2 = o.compute_energy(psi)
+    m, _, _ = o.mass_energy(psi, None, 6.0)
+    assert r == pytest.approx(r2, rel=1e-15)
+    assert m == pytest.approx(s * (6.0 / N) ** dim, rel=1e-14)
+
+
 def test_kinetic_energy_closed_form():
     N, L = 16, 4.0
     grid = np.indices((N, N))
