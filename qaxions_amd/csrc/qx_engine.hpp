@@ -34,7 +34,7 @@ struct Engine {
                           unsigned long long *vmax, hipStream_t s, const void *tiles = nullptr,
                           const double2 *ptab = nullptr, const float2 *ftab = nullptr,
                           long long ptab_stride = 0, double inv_n = 0.0) = 0;
-    void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N
+    void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N (FP32: then their low words, see cmul2)
     // streaming loads and stores in the passes (lattices whose arrays are far
     // larger than the memory-side cache; instantiated for N >= 512 only)
     bool stream_io = false;

@@ -26,6 +26,41 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
     return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
+// a * (wh + wl): the twiddle in two words, wh the rounded twiddle and wl =
+// w - wh rounded, so the product carries the twiddle to ~2^-48 instead of
+// the ~2^-25 of wh alone.  A rounded twiddle is the same operator error in
+// every transform, and the drift applies its transforms to psi step after
+// step: with one word the error of psi grows linearly with the step count;
+// with two words only the (random) rounding of the arithmetic is left, which
+// grows like its root.  a wl enters the fused multiply-adds that form a wh:
+// added to a product rounded first, it would mostly round away (it is below
+// half an ulp of it).
+__device__ __forceinline__ float2 cmul2(float2 a, float2 wh, float2 wl)
+{
+    const float cr = fmaf(a.x, wl.x, -a.y * wl.y), ci = fmaf(a.x, wl.y, a.y * wl.x);
+    return float2{fmaf(a.x, wh.x, fmaf(-a.y, wh.y, cr)), fmaf(a.x, wh.y, fmaf(a.y, wh.x, ci))};
+}
+__device__ __forceinline__ double2 cmul2(double2 a, double2 wh, double2) { return cmul(a, wh); }
+
+// Twiddle tables in LDS.  TW_PLAIN: tw[m], one word per twiddle.  FP32
+// passes that transform psi keep them interleaved, tw[2m] the rounded
+// twiddle and tw[2m + 1] its low word (one 16-byte read per twiddle):
+// TW_HI reads only the rounded twiddles of such a table, TW_TWO both words
+// (two-word products, cmul2).
+enum { TW_PLAIN = 0, TW_HI = 1, TW_TWO = 2 };
+template <int TWM, int DIR, typename C>
+__device__ __forceinline__ C twiddle_mul(C u, const C *tw, int m)
+{
+    if constexpr (TWM == TW_TWO && std::is_same<C, float2>::value) {
+        const float4 w4 = reinterpret_cast<const float4 *>(tw)[m];
+        const float2 wh{w4.x, DIR > 0 ? -w4.y : w4.y}, wl{w4.z, DIR > 0 ? -w4.w : w4.w};
+        return cmul2(u, wh, wl);
+    } else {
+        C w = tw[TWM == TW_PLAIN ? m : 2 * m];
+        if (DIR > 0) w.y = -w.y;
+        return cmul(u, w);
+    }
+}
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
 
@@ -143,7 +178,8 @@ template <int N, typename L, int EV> __device__ __forceinline__ void line_sync()
 
 // One Stockham stage.  v holds x[t + q TPL]; on exit (not last) v holds the
 // next stage's inputs, read back from LDS; (last) v holds y[t + q TPL].
-template <int N, int S, int DIR, typename L, typename C, int EV>
+// TWM: the twiddle table's layout and use (TW_PLAIN, TW_HI, TW_TWO).
+template <int N, int S, int DIR, typename L, int TWM, typename C, int EV>
 __device__ __forceinline__ void fft_stage(C (&v)[EV], int t, C *line, int tc, const C *tw)
 {
     using P = Plan<N, EV>;
@@ -161,11 +197,7 @@ __device__ __forceinline__ void fft_stage(C (&v)[EV], int t, C *line, int tc, co
         for (int r = 0; r < R; ++r) u[r] = v[m + r * NB];
         if (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                C w = tw[r * k * TWS];
-                if (DIR > 0) w.y = -w.y;
-                u[r] = cmul(u[r], w);
-            }
+            for (int r = 1; r < R; ++r) u[r] = twiddle_mul<TWM, DIR>(u[r], tw, r * k * TWS);
         }
         Dft<R, DIR, C>::run(u);
         if (last) {
@@ -188,12 +220,12 @@ __device__ __forceinline__ void fft_stage(C (&v)[EV], int t, C *line, int tc, co
     }
 }
 
-template <int N, int S, int DIR, typename L, typename C, int EV>
+template <int N, int S, int DIR, typename L, int TWM, typename C, int EV>
 __device__ __forceinline__ void stage_loop(C (&v)[EV], int t, C *line, int tc, const C *tw)
 {
     if constexpr (S < Plan<N, EV>::NST) {
-        fft_stage<N, S, DIR, L>(v, t, line, tc, tw);
-        stage_loop<N, S + 1, DIR, L>(v, t, line, tc, tw);
+        fft_stage<N, S, DIR, L, TWM>(v, t, line, tc, tw);
+        stage_loop<N, S + 1, DIR, L, TWM>(v, t, line, tc, tw);
     }
 }
 
@@ -201,18 +233,19 @@ __device__ __forceinline__ void stage_loop(C (&v)[EV], int t, C *line, int tc, c
 // must call it (column lines and lines of more than 64 threads synchronise
 // with block barriers).  `line` is this line's LDS
 // area, `tc` the column interleave for LdsCols, `tw` the LDS twiddle table
-// tw[m] = e^{-2 pi i m/N}.
-template <int N, int DIR, typename L, typename C, int EV>
+// tw[m] = e^{-2 pi i m/N} (TW_PLAIN), or that table interleaved with the
+// twiddles' low words (TW_HI, TW_TWO).
+template <int N, int DIR, typename L, int TWM = TW_PLAIN, typename C, int EV>
 __device__ __forceinline__ void fft_line(C (&v)[EV], int t, C *line, int tc, const C *tw)
 {
-    stage_loop<N, 0, DIR, L>(v, t, line, tc, tw);
+    stage_loop<N, 0, DIR, L, TWM>(v, t, line, tc, tw);
 }
 
 // Two lines transformed in lockstep (same direction): each stage's
 // butterflies of both lines, one synchronisation per exchange for both, two
 // independent dependency chains for the scheduler.  line_a / line_b are the
 // two lines' LDS areas.
-template <int N, int S, int DIR, typename L, typename C, int EV>
+template <int N, int S, int DIR, typename L, int TWM, typename C, int EV>
 __device__ __forceinline__ void fft_stage2(C (&va)[EV], C (&vb)[EV], int t, C *line_a,
                                            C *line_b, int tc, const C *tw)
 {
@@ -235,10 +268,8 @@ __device__ __forceinline__ void fft_stage2(C (&va)[EV], C (&vb)[EV], int t, C *l
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                C w = tw[r * k * TWS];
-                if (DIR > 0) w.y = -w.y;
-                ua[r] = cmul(ua[r], w);
-                ub[r] = cmul(ub[r], w);
+                ua[r] = twiddle_mul<TWM, DIR>(ua[r], tw, r * k * TWS);
+                ub[r] = twiddle_mul<TWM, DIR>(ub[r], tw, r * k * TWS);
             }
         }
         Dft<R, DIR, C>::run(ua);
@@ -275,21 +306,21 @@ __device__ __forceinline__ void fft_stage2(C (&va)[EV], C (&vb)[EV], int t, C *l
     }
 }
 
-template <int N, int S, int DIR, typename L, typename C, int EV>
+template <int N, int S, int DIR, typename L, int TWM, typename C, int EV>
 __device__ __forceinline__ void stage_loop2(C (&va)[EV], C (&vb)[EV], int t, C *line_a,
                                             C *line_b, int tc, const C *tw)
 {
     if constexpr (S < Plan<N, EV>::NST) {
-        fft_stage2<N, S, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
-        stage_loop2<N, S + 1, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
+        fft_stage2<N, S, DIR, L, TWM>(va, vb, t, line_a, line_b, tc, tw);
+        stage_loop2<N, S + 1, DIR, L, TWM>(va, vb, t, line_a, line_b, tc, tw);
     }
 }
 
-template <int N, int DIR, typename L, typename C, int EV>
+template <int N, int DIR, typename L, int TWM = TW_PLAIN, typename C, int EV>
 __device__ __forceinline__ void fft_line2(C (&va)[EV], C (&vb)[EV], int t, C *line_a,
                                           C *line_b, int tc, const C *tw)
 {
-    stage_loop2<N, 0, DIR, L>(va, vb, t, line_a, line_b, tc, tw);
+    stage_loop2<N, 0, DIR, L, TWM>(va, vb, t, line_a, line_b, tc, tw);
 }
 
 }  // namespace qx

@@ -95,6 +95,10 @@ template <int N, typename R> struct Geom {
     static constexpr int STPC = TLC >= 8 ? TLC + 1 : TLC;
     static constexpr int STGC = (N / 2 + 1) * STPC;
     static constexpr int ROW_LDS = XCH > STGC ? XCH : STGC;
+    // occupancy kept for the latency-limited single-wave-line c2r_rows
+    // (FP32): 5 waves per SIMD, as its registers allowed before the drift's
+    // two-word twiddles
+    static constexpr int C2R_WAVES = (WAVE_LINE && sizeof(C) == 8) ? 5 : 1;
 };
 
 // pointwise factor in fused passes
@@ -332,6 +336,15 @@ __device__ __forceinline__ void load_tw(C *tw, const C *twg)
 {
     for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
 }
+// FP32 twiddles with their low words (twg[N + m], see cmul2), interleaved
+// for TW_HI / TW_TWO transforms: tw[2m], tw[2m + 1]
+__device__ __forceinline__ void load_tw2(float2 *tw, const float2 *twg, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float2 h = twg[i], l = twg[n + i];
+        reinterpret_cast<float4 *>(tw)[i] = float4{h.x, h.y, l.x, l.y};
+    }
+}
 
 // ---------------------------------------------------------------------------
 // contiguous lines: data[line*N + pos], LPB lines per block.  With `out`, the
@@ -406,8 +419,22 @@ enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 // 16-column tile's 128-byte row segments with 512 threads per block).
 // NT: streaming accesses where a block row is a whole 128 B line (narrower
 // rows share lines between blocks, which then merge in L2)
+// Waves per SIMD that a strided FP32 drift block's LDS allows: its two-word
+// twiddles raise the register count, and the hint keeps it within what that
+// occupancy leaves (1 = no hint for every other pass)
+template <int N, typename R, int OP, int TC, int CPT> struct StridedWaves {
+    static constexpr bool ON = OP == OP_DRIFT_AXIS && std::is_same<R, float>::value;
+    // exchange area and two-word twiddles; the FP32 drift table where it fits
+    static constexpr long long LDS0 = 8ll * N * (TC / CPT) + 16ll * N;
+    static constexpr bool FTAB_LDS = LDS0 + 8ll * N <= 81920;
+    static constexpr long long LDS = LDS0 + (FTAB_LDS ? 8ll * N : 0);
+    static constexpr int W0 = (int)(163840 / LDS) * ((TC / CPT) * Plan<N>::TPL / 64) / 4;
+    static constexpr int value = !ON ? 1 : (W0 < 1 ? 1 : (W0 > 8 ? 8 : W0));
+};
+
 template <int N, typename R, int OP, int TC, int XM = XM_NONE, int CPT = 1, int TT = TC, bool NT = false>  // OP_FWD, OP_INV, OP_DRIFT_AXIS or OP_POISSON
 __global__ void __launch_bounds__(((TC / CPT) * Plan<N>::TPL))
+__attribute__((amdgpu_waves_per_eu(StridedWaves<N, R, OP, TC, CPT>::value)))
 strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long ntiles,
             MulArgs mul, const typename cplx<R>::T *__restrict__ twg,
             typename cplx<R>::T *__restrict__ tiles, long long outer)
@@ -415,14 +442,21 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     using C = typename cplx<R>::T;
     using P = Plan<N>;
     constexpr int TW = TC / CPT;
-    __shared__ C lds[N * TW];
-    __shared__ C tw[N];
-    // the drift's phase table is staged in LDS when small (N <= 1024); an
-    // FP32 single-axis drift stages its FP32 table
+    // FP32 drift: the transforms act on psi every step and take two-word
+    // twiddles (TW_TWO), interleaved in one table
     constexpr bool FAX = OP == OP_DRIFT_AXIS && std::is_same<R, float>::value;
-    constexpr bool PT_LDS = OP == OP_DRIFT_AXIS && N <= 1024;
+    constexpr int TWM = FAX ? TW_TWO : TW_PLAIN;
+    __shared__ C lds[N * TW];
+    __shared__ __align__(16) C tw[FAX ? 2 * N : N];
+    // the drift's phase table is staged in LDS when small: an FP32
+    // single-axis drift stages its FP32 table where two blocks still fit a
+    // CU (otherwise each thread reads its entries from global memory, where
+    // they stay cached; loading them ahead into registers would cost the
+    // second block)
+    constexpr bool PT_LDS = OP == OP_DRIFT_AXIS && (FAX ? StridedWaves<N, R, OP, TC, CPT>::FTAB_LDS : N <= 1024);
     __shared__ double2 ptab_lds[PT_LDS ? (FAX ? N / 2 : N) : 1];
-    load_tw<N>(tw, twg);
+    if constexpr (FAX) load_tw2((float2 *)tw, (const float2 *)twg, N);
+    else load_tw<N>(tw, twg);
     const int c = threadIdx.x % TW, t = threadIdx.x / TW;
     const long long b = xcd_block();
     const long long o = b / ntiles;
@@ -460,7 +494,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
         } else if constexpr (OP == OP_INV) {
             fft_line<N, +1, LdsCols>(v[j], t, lds + c, TW, tw);
         } else {
-            fft_line<N, -1, LdsCols>(v[j], t, lds + c, TW, tw);
+            fft_line<N, -1, LdsCols, TWM>(v[j], t, lds + c, TW, tw);
             const int cc = ok ? (int)col : 0;
             if constexpr (FAX) {
                 const float s = (float)ma.inv_s;
@@ -479,7 +513,7 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
                     v[j][q] = poisson_factor<R>(ma, n0 * n0 + mc, v[j][q]);
                 }
             }
-            fft_line<N, +1, LdsCols>(v[j], t, lds + c, TW, tw);
+            fft_line<N, +1, LdsCols, TWM>(v[j], t, lds + c, TW, tw);
         }
     }
 #pragma unroll
@@ -607,6 +641,7 @@ enum {
 
 template <int N, typename R, bool NT>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS_C))
+__attribute__((amdgpu_waves_per_eu(Geom<N, R>::C2R_WAVES)))
 c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
          typename cplx<R>::T *__restrict__ psi, long long rows, long long rpm, R kick_dt,
          const double *__restrict__ kick_member, int flags,
@@ -619,9 +654,15 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     using G = Geom<N, R>;
     constexpr int TL = G::TLC, H1 = N / 2 + 1;
     __shared__ C lds[G::ROW_LDS];
-    __shared__ C tw[N];
+    // FP32: twiddles with their low words, interleaved; the drift's
+    // transforms act on psi every step and take both (TW_TWO), the
+    // potential's transform the rounded twiddles alone (TW_HI)
+    constexpr bool XT = std::is_same<R, float>::value;
+    constexpr int TWH = XT ? TW_HI : TW_PLAIN, TWD = XT ? TW_TWO : TW_PLAIN;
+    __shared__ __align__(16) C tw[XT ? 2 * N : N];
     __shared__ double wmax[16];
-    load_tw<N>(tw, twg);
+    if constexpr (XT) load_tw2((float2 *)tw, (const float2 *)twg, N);
+    else load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
     // with 8-row blocks a block reads half of each 128 B line of vt: the
     // block holding the other half runs next on the same XCD
@@ -676,7 +717,7 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             z[q] = lo ? C{xa.x - xb.y, xa.y + xb.x} : C{xa.x + xb.y, xb.x - xa.y};
         }
         __syncthreads();
-        fft_line<N, +1, LdsContig>(z, t, mine, 1, tw);
+        fft_line<N, +1, LdsContig, TWH>(z, t, mine, 1, tw);
 #pragma unroll
         for (int q = 0; q < P::E; ++q) { va[q] = z[q].x; vb[q] = z[q].y; }
     } else if (flags & (C2R_STORE_V | C2R_VMAX | C2R_KICK)) {
@@ -729,10 +770,10 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             C *mine_b = lds + (G::NPC + p) * P::PADDED;
             __syncthreads();
             if constexpr (G::TWO_LINES) {
-                fft_line2<N, -1, LdsContig>(a, b, t, mine, mine_b, 1, tw);
+                fft_line2<N, -1, LdsContig, TWD>(a, b, t, mine, mine_b, 1, tw);
             } else {
-                fft_line<N, -1, LdsContig>(a, t, mine, 1, tw);
-                fft_line<N, -1, LdsContig>(b, t, mine, 1, tw);
+                fft_line<N, -1, LdsContig, TWD>(a, t, mine, 1, tw);
+                fft_line<N, -1, LdsContig, TWD>(b, t, mine, 1, tw);
             }
             if constexpr (std::is_same<R, float>::value) {
                 const float2 *fa = ftab + ma_, *fb = ftab + mb_;
@@ -754,10 +795,10 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
                 }
             }
             if constexpr (G::TWO_LINES) {
-                fft_line2<N, +1, LdsContig>(a, b, t, mine, mine_b, 1, tw);
+                fft_line2<N, +1, LdsContig, TWD>(a, b, t, mine, mine_b, 1, tw);
             } else {
-                fft_line<N, +1, LdsContig>(a, t, mine, 1, tw);
-                fft_line<N, +1, LdsContig>(b, t, mine, 1, tw);
+                fft_line<N, +1, LdsContig, TWD>(a, t, mine, 1, tw);
+                fft_line<N, +1, LdsContig, TWD>(b, t, mine, 1, tw);
             }
         }
 #pragma unroll

@@ -62,7 +62,10 @@ std::unique_ptr<Engine> make_engine(int N, bool fp64) { return fp64 ? make_engin
 // the pair of neighbouring floats (cos and sin each rounded down or up) whose
 // squared modulus is closest to 1: nearest rounding of the two parts leaves
 // |w|^2 - 1 with a nonzero mean over the table (-4e-9 at N = 256), a bias
-// every twiddled multiply would add to the norm (mass) of psi.
+// every twiddled multiply would add to the norm (mass) of psi.  FP32 tables
+// hold N more entries after those: the remainders w - wh of the rounded
+// twiddles wh, rounded, which the drift's transforms add in (two-word
+// products, cmul2).
 qx_status upload_twiddles(void *dst, int N, bool fp64)
 {
     std::vector<double> tw(2 * (size_t)N);
@@ -79,7 +82,7 @@ qx_status upload_twiddles(void *dst, int N, bool fp64)
     if (fp64) {
         e = hipMemcpy(dst, tw.data(), 16 * (size_t)N, hipMemcpyHostToDevice);
     } else {
-        std::vector<float> tf(2 * (size_t)N);
+        std::vector<float> tf(4 * (size_t)N);
         for (int m = 0; m < N; ++m) {
             const double cr = tw[2 * m], ci = tw[2 * m + 1];
             float bc = (float)cr, bs = (float)ci;
@@ -94,8 +97,12 @@ qx_status upload_twiddles(void *dst, int N, bool fp64)
                 }
             tf[2 * m] = bc;
             tf[2 * m + 1] = bs;
+            // low words: the remainder of each part, for two-word products
+            // (cmul2)
+            tf[2 * (size_t)N + 2 * m] = (float)(cr - (double)bc);
+            tf[2 * (size_t)N + 2 * m + 1] = (float)(ci - (double)bs);
         }
-        e = hipMemcpy(dst, tf.data(), 8 * (size_t)N, hipMemcpyHostToDevice);
+        e = hipMemcpy(dst, tf.data(), 16 * (size_t)N, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) return fail(QX_EDEVICE, "twiddle upload failed");
     return QX_OK;
@@ -589,7 +596,7 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
     if ((s = ensure(c.get(), &c->psi, (size_t)c->S * c->csz)) != QX_OK) return cleanup(s);
     if ((s = ensure(c.get(), &c->V, (size_t)c->S * c->rsz)) != QX_OK) return cleanup(s);
     if ((s = ensure(c.get(), &c->vt, (size_t)c->H * c->csz)) != QX_OK) return cleanup(s);
-    if ((s = ensure(c.get(), &c->eng->tw, (size_t)N * c->csz)) != QX_OK) return cleanup(s);
+    if ((s = ensure(c.get(), &c->eng->tw, (size_t)N * 16)) != QX_OK) return cleanup(s);
     if ((s = ensure(c.get(), (void **)&c->partials, sizeof(double) * 2 * MAX_PART)) != QX_OK)
         return cleanup(s);
     if ((s = ensure(c.get(), (void **)&c->vmax_bits, sizeof(unsigned long long))) != QX_OK)
@@ -1630,7 +1637,7 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
         void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_ptab, (void **)&k.d_pfac,
                          (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
         const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
-                             (size_t)(k.B * k.H) * e->csz, (size_t)k.N * e->csz,
+                             (size_t)(k.B * k.H) * e->csz, (size_t)k.N * 16,
                              (sizeof(double2) + sizeof(float2)) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
                              sizeof(double) * k.B, sizeof(double) * k.B};
         for (int b = 0; b < 9; ++b)
