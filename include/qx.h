@@ -145,6 +145,12 @@ qx_status qx_propagate(qx_ctx *ctx, double ds);
 qx_status qx_fft_c2c(qx_ctx *ctx, double *data_c128, int sign);
 /* forward_r2c (src/fft/fft.h:132-135): S real -> H complex half spectrum */
 qx_status qx_fft_r2c(qx_ctx *ctx, const double *in, double *out_c128);
+/* forward_r2c of the context's own V, as Field::updatePotential
+ * (src/field.cxx:312) and Spectrum::compute (src/spectrum/spectrum.cxx:48)
+ * call it on the Field's V: H complex half spectrum (reference layout),
+ * times `scale` (1/S gives the Poisson-multiplied spectrum that the last
+ * potential update transformed back into V) */
+qx_status qx_fft_r2c_V(qx_ctx *ctx, double scale, double *out_c128);
 /* backward_c2r (src/fft/fft.h:137-140): H complex -> S real; `in` is not
  * modified (FFTW may destroy it, so callers cannot rely on either) */
 qx_status qx_fft_c2r(qx_ctx *ctx, const double *in_c128, double *out);

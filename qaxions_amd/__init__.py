@@ -78,6 +78,7 @@ SIGNATURES = {
     "qx_fft_c2c": (C.c_int, [C.c_void_p, _dp, C.c_int]),
     "qx_fft_r2c": (C.c_int, [C.c_void_p, _dp, _dp]),
     "qx_fft_c2r": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "qx_fft_r2c_V": (C.c_int, [C.c_void_p, C.c_double, _dp]),
     "qx_sites": (C.c_int64, [C.c_void_p]),
     "qx_ksites": (C.c_int64, [C.c_void_p]),
     "qx_global_N": (C.c_int, [C.c_void_p]),
@@ -336,6 +337,12 @@ class Lattice:
         a = np.ascontiguousarray(x, dtype=np.float64)
         out = np.empty(self.half_shape, np.complex128)
         _check(lib().qx_fft_r2c(self._h, _p(a), _p(out.view(np.float64))))
+        return out
+
+    def fft_r2c_V(self, scale=1.0):
+        """forward r2c of the lattice's own V, times scale (reference layout)"""
+        out = np.empty(self.half_shape, np.complex128)
+        _check(lib().qx_fft_r2c_V(self._h, float(scale), _p(out.view(np.float64))))
         return out
 
     def fft_c2r(self, xh):

@@ -1278,6 +1278,22 @@ qx_status qx_fft_r2c(qx_ctx *c, const double *in, double *out)
     return download_c(c, c->aux_c, out, c->H);
 }
 
+qx_status qx_fft_r2c_V(qx_ctx *c, double scale, double *out)
+{
+    CHECK_CTX(c);
+    if (!out) return fail(QX_EINVAL, "null output");
+    TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
+    TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
+    c->eng->r2c_rows(R2C_REAL, nullptr, c->V, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
+    vt_forward(c, c->aux_vt);
+    TRY(vt_to_ref_launch(c, c->aux_vt, c->aux_c));
+    TRY(sync(c));
+    TRY(download_c(c, c->aux_c, out, c->H));
+    if (scale != 1.0)
+        for (long long i = 0; i < 2 * c->H; ++i) out[i] *= scale;
+    return QX_OK;
+}
+
 qx_status qx_fft_c2r(qx_ctx *c, const double *in, double *out)
 {
     CHECK_CTX(c);

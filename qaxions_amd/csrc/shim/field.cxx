@@ -67,6 +67,7 @@ void Field::init(Params &p)
     psi_h_.assign(2 * sites_, 0.0);
     V_h_.assign(sites_, 0.0);
     Vhat_.assign(2 * ksites_, 0.0);
+    vhat_src_ = VhatSrc::Host;
     psi_host_valid_ = V_host_valid_ = true;
     psi_host_dirty_ = V_host_dirty_ = false;
     setPhysics();
@@ -141,8 +142,38 @@ const double *Field::V() const
 double *Field::V()
 {
     const_cast<const Field *>(this)->V();
+    beforeVChange();
     V_host_dirty_ = true;
     return V_h_.data();
+}
+
+qx_complex *Field::Vhat()
+{
+    if (vhat_src_ != VhatSrc::Host) {
+        syncToDevice();
+        const double scale = vhat_src_ == VhatSrc::PoissonOfV ? 1.0 / (double)sites_ : 1.0;
+        qx_throw(qx_fft_r2c_V(ctx_, scale, Vhat_.data()));
+        vhat_src_ = VhatSrc::Host;
+    }
+    return (qx_complex *)Vhat_.data();
+}
+
+void Field::beforeVChange() const
+{
+    if (vhat_src_ == VhatSrc::R2cOfV) const_cast<Field *>(this)->Vhat();
+    vhat_src_ = VhatSrc::Host;
+}
+
+void Field::beginDeviceWrite()
+{
+    syncToDevice();
+    beforeVChange();
+}
+
+void Field::endDeviceWrite()
+{
+    psi_host_valid_ = false;
+    V_host_valid_ = false;
 }
 
 // src/field.cxx:93-116
@@ -166,8 +197,9 @@ void Field::drift(double dt)
 // src/field.cxx:230-291: Poisson factor applied to Vhat() in place
 void Field::Poisson()
 {
+    double *vh = (double *)Vhat();
     setPhysics();
-    qx_throw(qx_poisson_half(ctx_, Vhat_.data()));
+    qx_throw(qx_poisson_half(ctx_, vh));
 }
 
 // src/field.cxx:294-346
@@ -179,12 +211,14 @@ void Field::updatePotential()
     qx_throw(qx_update_potential(ctx_, &vmax));
     Vmax_ = vmax;
     V_host_valid_ = false;
+    vhat_src_ = VhatSrc::PoissonOfV;
 }
 
 // src/field.cxx:200-227: overwrites V with |psi|^2 and sets rhomax
 void Field::computeEnergy()
 {
     syncToDevice();
+    beforeVChange();
     double r = 0.0;
     qx_throw(qx_compute_energy(ctx_, &r));
     rhomax_ = r;
@@ -245,6 +279,7 @@ void Field::leapfrog(int nsteps, bool first_half, bool last_half)
         Vmax_ = qx_vmax(ctx_);
         psi_host_valid_ = false;
         V_host_valid_ = false;
+        vhat_src_ = VhatSrc::PoissonOfV;
         return;
     }
     syncToDevice();
@@ -253,5 +288,8 @@ void Field::leapfrog(int nsteps, bool first_half, bool last_half)
     for (int i = 0; i < nsteps; ++i) updateTime();
     Vmax_ = qx_vmax(ctx_);
     psi_host_valid_ = false;
-    V_host_valid_ = false;
+    if (nsteps > 0) {
+        V_host_valid_ = false;
+        vhat_src_ = VhatSrc::PoissonOfV;
+    }
 }

@@ -1,8 +1,17 @@
 // field.h -- the reference's Field (src/field.h:13-97) with its state on the
 // GPU.  Public methods keep the reference's names, arguments and effects;
-// psi()/V() hand out host mirrors that are synchronised lazily: a mutable
-// accessor marks the mirror dirty and the next operator uploads it, and a
-// device-side update is downloaded on the next access.
+// psi()/V()/Vhat() hand out host mirrors that are synchronised lazily: a
+// mutable accessor of psi or V marks the mirror dirty and the next operator
+// uploads it, and a device-side update is downloaded on the next access.
+//
+// Vhat holds what the reference's Vhat_ holds: r2c(V) after
+// Spectrum::compute (src/spectrum/spectrum.cxx:48), and after
+// updatePotential the Poisson-multiplied spectrum that was transformed back
+// into V (src/field.cxx:312-332; the reference's buffer is then whatever
+// FFTW's c2r left in its input, which is not specified, so here it holds the
+// spectrum itself).  Both are formed from the device's V when Vhat() is
+// read: a spectrum of V is downloaded before anything rewrites V, the
+// Poisson spectrum is dropped then (the reference has none to keep).
 #ifndef QX_SHIM_FIELD_H
 #define QX_SHIM_FIELD_H
 
@@ -44,7 +53,7 @@ class Field {
     double *V();
     double getVIdx(int idx) const { return V()[idx]; }
 
-    qx_complex *Vhat() { return Vhat_.empty() ? nullptr : (qx_complex *)Vhat_.data(); }
+    qx_complex *Vhat();
 
     FFTBackend *fftBackend() const { return fft_backend_.get(); }
 
@@ -67,10 +76,20 @@ class Field {
     qx_ctx *context() const { return ctx_; }
     // push host edits to the device (called by every device operator)
     void syncToDevice() const;
+    // plumbing for the shim's own callers (no reference counterpart):
+    // Spectrum::compute has transformed V, so Vhat is r2c(V)
+    void vhatIsSpectrumOfV() { vhat_src_ = VhatSrc::R2cOfV; }
+    // a device-side writer (an initial condition) is about to rewrite psi
+    // and V: pending host edits go first, the host mirrors are refreshed on
+    // the next access
+    void beginDeviceWrite();
+    void endDeviceWrite();
 
   private:
     void setPhysics();
     void releaseContext();
+    // called before the device's V is rewritten (see Vhat above)
+    void beforeVChange() const;
 
     int N_ = 0;
     int dim_ = 0;
@@ -96,7 +115,11 @@ class Field {
     std::unique_ptr<FFTBackend> fft_backend_;
     // host mirrors
     mutable std::vector<double> psi_h_, V_h_;
-    std::vector<double> Vhat_;
+    mutable std::vector<double> Vhat_;
+    // where the current Vhat lives: the host array, or to be formed from the
+    // device's V as r2c(V) or as r2c(V)/S (the Poisson spectrum)
+    enum class VhatSrc { Host, R2cOfV, PoissonOfV };
+    mutable VhatSrc vhat_src_ = VhatSrc::Host;
     mutable bool psi_host_valid_ = true, psi_host_dirty_ = false;
     mutable bool V_host_valid_ = true, V_host_dirty_ = false;
 };

@@ -6,6 +6,7 @@
 #include <pybind11/pybind11.h>
 #include "field.h"
 #include "ic.h"
+#include "spectrum.h"
 
 #include <complex>
 
@@ -88,8 +89,44 @@ py::tuple mre_leapfrog(int n, int N, int nsteps, bool fused, bool fp32)
     return py::make_tuple(out, field->a(), field->ds(), field->time());
 }
 
+namespace {
+py::array_t<std::complex<double>> complex_array(const qx_complex *p, size_t n)
+{
+    py::array_t<std::complex<double>> out(n);
+    auto *o = out.mutable_data();
+    for (size_t i = 0; i < n; ++i) o[i] = std::complex<double>(p[i][0], p[i][1]);
+    return out;
+}
+}  // namespace
+
+// Not in the reference module: the Field's Vhat() through the reference's
+// sequences (soliton IC): after updatePotential (the Poisson spectrum the
+// potential was formed from); after computeEnergy and Spectrum::compute
+// (r2c of V = |psi|^2, src/spectrum/spectrum.cxx:48); then after
+// Field::Poisson() applied to that spectrum in place.  Returns (psi, the
+// three half spectra, V after updatePotential).
+py::tuple vhat_sequence(int n, int N, bool fp32)
+{
+    auto field = make_field(n, N, fp32);
+    const size_t S = field->sites(), H = field->ksites();
+    auto psi = complex_array(static_cast<const Field &>(*field).psi(), S);
+    field->updatePotential();
+    py::array_t<double> V(S);
+    const double *v = static_cast<const Field &>(*field).V();
+    std::copy(v, v + S, V.mutable_data());
+    auto after_potential = complex_array(field->Vhat(), H);
+    field->computeEnergy();
+    Spectrum spec(*field);
+    spec.compute();
+    auto after_spectrum = complex_array(field->Vhat(), H);
+    field->Poisson();
+    auto after_poisson = complex_array(field->Vhat(), H);
+    return py::make_tuple(psi, after_potential, after_spectrum, after_poisson, V);
+}
+
 PYBIND11_MODULE(fieldtest, m)
 {
+    m.def("vhat_sequence", &vhat_sequence, "", py::arg("n"), py::arg("N"), py::arg("fp32") = false);
     m.def("test_kick", &test_kick, "", py::arg("n"), py::arg("N"), py::arg("dt"), py::arg("idx"),
           py::arg("fp32") = false);
     m.def("test_drift", &test_drift, "", py::arg("n"), py::arg("N"), py::arg("dt"), py::arg("idx"),

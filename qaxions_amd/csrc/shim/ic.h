@@ -13,8 +13,8 @@ class InitialCondition {
     virtual std::string name() const = 0;
 };
 
-// Gaussian solitons read from ./solitons.txt, evaluated on the host in FP64
-// as the reference does (src/ic/solitons.cxx:20-110)
+// Gaussian solitons from the table ./solitons.txt, formed on the device
+// (src/ic/solitons.cxx:20-110)
 class SolitonsIC : public InitialCondition {
   public:
     explicit SolitonsIC(const Params &p) : p_(p) {}

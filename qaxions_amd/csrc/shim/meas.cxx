@@ -27,16 +27,10 @@ void measure(Field &f, const Params &p, size_t measNumber, Clock::time_point st,
     if (meas & MeasureType::SPECTRUM) {
         if (f.verb()) std::cout << "[measure] Saving power spectrum" << std::endl;
         Spectrum spec(f);
-        if (legacy_clobber) {
-            spec.compute();
-        } else {
-            // spectrum of |psi|^2 without writing V
-            const int n = f.size() / 2 + 1;
-            std::vector<double> P(n), k(n);
-            std::vector<int64_t> c(n);
-            qx_throw(qx_spectrum(f.context(), 1, n, 1, P.data(), c.data(), k.data()));
-            spec.compute();  // keeps the object's arrays consistent with V
-        }
+        if (legacy_clobber)
+            spec.compute();          // V holds |psi|^2 after computeEnergy
+        else
+            spec.computeDensity();   // |psi|^2 formed on the device, V untouched
         std::ostringstream fname;
         fname << p.dir << "/spec_" << std::setw(4) << std::setfill('0') << measNumber << ".txt";
         spec.write(fname.str());

@@ -18,11 +18,21 @@ void Spectrum::compute() { compute(N_ / 2 + 1, 1); }
 
 void Spectrum::compute(int nbins, int spb)
 {
+    run(0, nbins, spb);
+    f_.vhatIsSpectrumOfV();
+}
+
+void Spectrum::computeDensity() { computeDensity(N_ / 2 + 1, 1); }
+
+void Spectrum::computeDensity(int nbins, int spb) { run(1, nbins, spb); }
+
+void Spectrum::run(int source, int nbins, int spb)
+{
     f_.syncToDevice();
     std::vector<int64_t> cnt(nbins);
     Pk_.assign(nbins, 0.0);
     k_.assign(nbins, 0.0);
-    qx_throw(qx_spectrum(f_.context(), 0, nbins, spb, Pk_.data(), cnt.data(), k_.data()));
+    qx_throw(qx_spectrum(f_.context(), source, nbins, spb, Pk_.data(), cnt.data(), k_.data()));
     count_.assign(cnt.begin(), cnt.end());
 }
 

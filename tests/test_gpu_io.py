@@ -7,6 +7,8 @@ import os
 import numpy as np
 import pytest
 
+import oracle as o
+
 pytestmark = pytest.mark.gpu
 
 JAXIONS_NORM = 0.01 ** 1.5 / 1.41421356237   # src/io/io.cxx:12,264-265
@@ -63,24 +65,49 @@ def test_background_snapshot_and_restart(h5, n, N, fp32, in_golden_dir, tmp_path
     assert rel_l2(out["psi_restart_final"], out["psi_final"]) < (2e-6 if fp32 else 1e-12)
 
 
-@pytest.mark.parametrize("legacy,background", [(True, False), (True, True), (False, False), (False, True)])
-def test_measure_writes_spectrum_and_grid(h5, legacy, background, in_golden_dir, tmp_path):
-    print(f"[case] measure legacy={legacy} background={background}", flush=True)
-    n, N = 2, 32
-    psi = h5.measure_run(n, N, str(tmp_path), 3, legacy_clobber=legacy, background=background)
+def reference_spectrum_text(a, k, P, count):
+    """Spectrum::write's text (src/spectrum/spectrum.cxx:167-183): the scale
+    factor in the stream's default format, the column header, then
+    `k  P  count` in std::scientific (6 digits) for the non-empty shells."""
+    out = f"# a: {a:g}\n# k_phys      P(k)          count\n"
+    for kb, Pb, cb in zip(k, P, count):
+        if cb > 0:
+            out += f"{kb:.6e}  {Pb:.6e}  {int(cb)}\n"
+    return out
+
+
+@pytest.mark.parametrize("legacy,background,fp32", [(True, False, False), (True, True, False),
+                                                    (False, False, False), (False, True, False),
+                                                    (True, False, True), (False, False, True)])
+def test_measure_writes_spectrum_and_grid(h5, legacy, background, fp32, in_golden_dir, tmp_path):
+    """measure() (src/meas.cxx:13-63) with SPECTRUM | RHO_GRID, with and
+    without the reference's clobbering of V: both spectrum files hold the
+    P(k) of |psi|^2 (the oracle's, shell by shell) in the reference's text
+    format, byte for byte (FP64; FP32 to the FP32 spectrum's accuracy)."""
+    print(f"[case] measure legacy={legacy} background={background} fp32={fp32}", flush=True)
+    n, N, L = 2, 32, 1.0
+    psi = h5.measure_run(n, N, str(tmp_path), 3, legacy_clobber=legacy, background=background, fp32=fp32)
     spec = tmp_path / "spec_0003.txt"
     grid = tmp_path / "field_0003.h5"
     assert spec.exists() and grid.exists()
-    lines = spec.read_text().splitlines()
-    assert lines[0].startswith("# a:")
+    Po, co, ko = o.spectrum(np.abs(psi.reshape((N,) * n)) ** 2, L)
+    text = spec.read_text()
+    lines = text.splitlines()
+    assert lines[0] == "# a: 1"
     rows = np.array([[float(x) for x in l.split()] for l in lines[2:]])
-    assert rows.shape[1] == 3 and np.all(rows[:, 2] > 0)
+    keep = co > 0
+    assert rows.shape == (keep.sum(), 3)
+    assert np.array_equal(rows[:, 2], co[keep])
+    assert np.allclose(rows[:, 0], ko[keep], rtol=5e-7, atol=0)
+    assert np.all(np.abs(rows[:, 1] - Po[keep]) <= (5e-7 if not fp32 else 2e-6) * Po[keep])
+    if not fp32:
+        assert text == reference_spectrum_text(1.0, ko, Po, co)
     hdr, _ = h5.read_conf_arrays(str(grid), with_psi=False)
     assert hdr["step"] == 1 and hdr["N"] == N
     rho = np.abs(psi) ** 2
     got = h5.read_rho(str(grid))
     if legacy:   # V after computeEnergy: |psi|^2 formed on the device
-        assert rel_l2(got, rho) < 1e-14
+        assert rel_l2(got, rho) < (1e-14 if not fp32 else 2e-7)
     else:        # formed on the host from psi
         assert np.array_equal(got, psi.real * psi.real + psi.imag * psi.imag)
 

@@ -25,6 +25,18 @@ def lattice(qx, dim, N, L=1.0, prec="fp64", **ph):
 
 TOL = {"fp64": 1e-12, "fp32": 2e-6}
 
+
+def spectrum_errors(P, Po, N, spb):
+    """Per-bin relative error max_b |P_b - Po_b| / Po_b over the bins whose
+    shells all lie inside the GRF IC's spectral cut (|n| <= N/4), and the
+    largest absolute error of the other bins relative to bin 0 (those hold
+    only what the evolution moved there, or round-off at the IC)."""
+    shells_hi = (np.arange(len(P)) + 1) * spb - 1
+    cut = shells_hi <= N // 4
+    rel = np.abs(P - Po)[cut] / Po[cut]
+    out = np.abs(P - Po)[~cut].max() / Po[0] if (~cut).any() else 0.0
+    return float(rel.max()), float(out)
+
 FFT_CASES = [(2, n) for n in (2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096)] + \
             [(3, n) for n in (2, 4, 8, 16, 32, 64, 128)]
 
@@ -140,16 +152,19 @@ def test_spectrum(qx, dim, N, nbins, spb, prec):
     Po, co, ko = o.spectrum(np.abs(psi) ** 2, L, nbins, spb)
     assert np.array_equal(c, co)
     assert np.allclose(k, ko, rtol=1e-15)
-    # bins beyond the IC's spectral cut hold only roundoff: compare with an
-    # absolute floor tied to the largest bin
-    rtol, afl = (1e-11, 1e-13) if prec == "fp64" else (2e-3, 1e-6)
-    floor = afl * Po.max()
-    assert np.all(np.abs(P - Po) <= rtol * Po + floor)
+    # bins inside the IC's spectral cut: per-bin relative error (FP32
+    # measured <= 1.4e-7 at 512^3); the bins beyond it hold round-off only
+    # (|psi|^2 of this IC is band-limited), compared against bin 0
+    rtol, otol = (1e-12, 1e-20) if prec == "fp64" else (1e-6, 1e-12)
+    rel, out = spectrum_errors(P, Po, N, spb)
+    print(f"P(k) {prec}: max rel err in the cut {rel:.2e}, outside {out:.2e}")
+    assert rel <= rtol and out <= otol
     # the reference's choice: spectrum of V as it stands after computeEnergy
     lat.compute_energy()
     P2, c2, _ = lat.spectrum(source="V", nbins=nbins, spb=spb)
     assert np.array_equal(c2, co)
-    assert np.all(np.abs(P2 - Po) <= rtol * Po + floor)
+    rel, out = spectrum_errors(P2, Po, N, spb)
+    assert rel <= rtol and out <= otol
 
 
 # ------------------------------------------------------------------ reference tests
@@ -181,6 +196,31 @@ def test_reference_ictest_module(N, idx, in_golden_dir):
         got = f(N, idx)
         assert got == pytest.approx(want, abs=1e-5, rel=1e-4)
         assert abs(got - want) < 1e-14
+
+
+@pytest.mark.parametrize("n,N", [(2, 8), (2, 16), (3, 8)])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_field_vhat_mirror(n, N, fp32, in_golden_dir):
+    """Field::Vhat() (src/field.h:45) through the reference's sequences:
+    after updatePotential the Poisson-multiplied spectrum of the density
+    (src/field.cxx:300-321); after computeEnergy and Spectrum::compute,
+    r2c of V = |psi|^2 (src/spectrum/spectrum.cxx:48); Field::Poisson()
+    then multiplies that spectrum in place (src/field.cxx:230-291)."""
+    import qaxions_amd.fieldtest as field
+    psi, vh_pot, vh_spec, vh_pois, V = field.vhat_sequence(n, N, fp32=fp32)
+    shape, half = (N,) * n, (N,) * (n - 1) + (N // 2 + 1,)
+    psi = psi.reshape(shape)
+    L, pref = 1.0, 4 * math.pi   # defaults(): Lbox = 1, norm = 1 (STATIC)
+    Vo, Vho, _ = o.update_potential(psi, L, pref)
+    # FP32: V's constant -pref(<rho> - 1) (the reference's k = 0 rule) is a
+    # cancellation and carries ~1e-6 absolute
+    tol = 1e-12 if not fp32 else 1e-5
+    assert rel_l2(V.reshape(shape), Vo) < tol
+    assert rel_l2(vh_pot.reshape(half), Vho) < tol
+    tol = 1e-12 if not fp32 else 2e-6
+    rho_hat = o.fft_r2c(np.abs(psi) ** 2)
+    assert rel_l2(vh_spec.reshape(half), rho_hat) < tol
+    assert rel_l2(vh_pois.reshape(half), o.poisson(rho_hat, L, 1)) < tol
 
 
 @pytest.mark.parametrize("n,N,idx", rm.PARAMS_POT)
@@ -345,23 +385,32 @@ def test_plain_method_256sq_100_steps(qx, prec, tol):
     print(f"psi rel L2 after {steps} steps ({prec}): {err:.3e}")
     assert err <= tol
     m0 = (np.abs(psi0) ** 2).sum()
-    # FP32: 100 steps of unitary FFT passes in single precision keep the mass
-    # to a few 1e-6 (round-off of ~4 transforms per step)
-    assert abs((np.abs(got) ** 2).sum() - m0) / m0 < (1e-13 if prec == "fp64" else 1e-5)
+    # FP32: the drift's transforms take two-word twiddles, so 100 steps of
+    # single-precision passes keep the mass to 3.4e-8 (one-word twiddles:
+    # ~1e-6, growing with every step); what is left is mostly the FP32
+    # rounding of the per-axis drift factors, the same every step
+    assert abs((np.abs(got) ** 2).sum() - m0) / m0 < (1e-13 if prec == "fp64" else 1e-7)
+    # 64-bin P(k) of |psi|^2 at the end against the oracle's: per-bin
+    # relative error inside the IC's cut (FP32 measured 8.3e-7: the state's
+    # own error, the spectrum of one state agrees to 1e-7)
     P, c, _ = lat.spectrum(source="rho", nbins=64, spb=2)
     Po, co, _ = o.spectrum(np.abs(want) ** 2, L, 64, 2)
     assert np.array_equal(c, co)
-    rtol, afl = (1e-10, 1e-13) if prec == "fp64" else (1e-3, 1e-6)
-    assert np.all(np.abs(P - Po) <= rtol * Po + afl * Po.max())
+    rel, out = spectrum_errors(P, Po, N, 2)
+    print(f"P(k) after {steps} steps ({prec}): max rel err in the cut {rel:.2e}, outside {out:.2e}")
+    assert rel <= (1e-12 if prec == "fp64" else 5e-6)
+    assert out <= (1e-16 if prec == "fp64" else 1e-10)
 
 
+@pytest.mark.parametrize("N", [32, 64])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
-def test_config1_blobs_vs_oracle(qx, prec, tol):
+def test_config1_blobs_vs_oracle(qx, N, prec, tol):
     """The 3D soliton-blob workload (32 seeded Gaussian blobs on background
-    0.1, L = 20, dt = 5e-4, 200 KDK steps, mean density subtracted) at 64^3,
-    where the FP64 oracle runs in seconds: psi against the oracle, and the
-    oracle's own mass and energy conservation reproduced."""
-    N, L, dt, steps = 64, 20.0, 5e-4, 200
+    0.1, L = 20, dt = 5e-4, 200 KDK steps, mean density subtracted) at 32^3
+    and 64^3, where the FP64 oracle runs in seconds: psi against the oracle
+    (FP32 measured 2.7e-6 and 2.9e-6), and the oracle's own mass and energy
+    conservation reproduced."""
+    L, dt, steps = 20.0, 5e-4, 200
     tab = o.blob_table(32, 7, L)
     psi0 = o.ic_blobs(3, N, L, tab, 0.1)
     V0, _, _ = o.update_potential(psi0, L, 1.0, 1.0, 0, 0)
@@ -374,40 +423,46 @@ def test_config1_blobs_vs_oracle(qx, prec, tol):
     err = rel_l2(lat.psi, want)
     print(f"psi rel L2 after {steps} steps ({prec}): {err:.3e}")
     assert err <= tol
-    assert rel_l2(lat.V, Vw) <= (1e-12 if prec == "fp64" else 2e-5)
+    assert rel_l2(lat.V, Vw) <= (1e-12 if prec == "fp64" else 5e-6)
     e1 = lat.energy()
-    assert abs(e1[0] - e0[0]) / e0[0] < (1e-13 if prec == "fp64" else 1e-5)
+    assert abs(e1[0] - e0[0]) / e0[0] < (1e-13 if prec == "fp64" else 1e-7)
 
 
 def test_config1_256cube_fp32_against_fp64(qx):
     """The same workload at its full 256^3 size: the FP32 run against the
-    FP64 run (pinned to the oracle above), mass and energy drift over 200
-    steps.  FP64 energy changes by the method's own 3e-9; FP32 adds round-off."""
+    FP64 run (pinned to the oracle above) after 100 and 200 steps, mass and
+    energy drift over 200 steps.  FP64 energy changes by the method's own
+    3e-9; FP32 adds round-off."""
     N, L, dt, steps = 256, 20.0, 5e-4, 200
     tab = o.blob_table(32, 7, L)
-    out, drift = {}, {}
+    out, mid, drift = {}, {}, {}
     for prec in ("fp64", "fp32"):
+        print(f"[case] 256^3 blobs {prec}", flush=True)
         lat = qx.Lattice(3, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
         lat.ic_blobs(tab, 0.1)
         lat.update_potential()
         e0 = lat.energy()
-        lat.leapfrog(dt, steps)
+        lat.leapfrog(dt, steps // 2, first_half=True, last_half=False)
+        mid[prec] = lat.psi
+        lat.leapfrog(dt, steps - steps // 2, first_half=False, last_half=True)
         e1 = lat.energy()
         out[prec] = lat.psi
         drift[prec] = (abs(e1[0] - e0[0]) / e0[0], abs((e1[1] + e1[2]) - (e0[1] + e0[2])) / abs(e0[1] + e0[2]))
         lat.close()
+    err_mid = rel_l2(mid["fp32"], mid["fp64"])
     err = rel_l2(out["fp32"], out["fp64"])
-    print(f"256^3 fp32 vs fp64 after {steps} steps: {err:.3e}; drifts {drift}")
-    assert err < 2e-5
+    print(f"256^3 fp32 vs fp64 after {steps // 2} steps: {err_mid:.3e}, after {steps}: {err:.3e}; drifts {drift}")
+    assert err_mid < 1e-5 and err < 1e-5
     assert drift["fp64"][0] < 1e-12 and drift["fp64"][1] < 1e-8
-    assert drift["fp32"][0] < 1e-5 and drift["fp32"][1] < 1e-4
+    assert drift["fp32"][0] < 1e-7 and drift["fp32"][1] < 1e-4
 
 
 def test_headline_512cube_fp32_against_fp64_100_steps(qx):
     """The headline workload at full size (3D 512^3, L = 40, dt = 2.5e-4,
     GRF seed 0): after 100 steps the FP32 state is within 1e-5 relative L2
-    of the FP64 run, whose path matches the FP64 oracle to 1e-12 at the
-    oracle's sizes; mass drift 1e-13 (FP64) and 1e-5 (FP32)."""
+    of the FP64 run, which matches the FP64 oracle to 1e-12 (the oracle
+    comparison at this size is test_headline_512cube_against_oracle); mass
+    drift 1e-13 (FP64) and 1e-8 (FP32)."""
     N, L, dt, steps = 512, 40.0, 2.5e-4, 100
     out, drift = {}, {}
     for prec in ("fp64", "fp32"):
@@ -424,7 +479,39 @@ def test_headline_512cube_fp32_against_fp64_100_steps(qx):
     err = rel_l2(out["fp32"], out["fp64"])
     print(f"512^3 fp32 vs fp64 after {steps} steps: {err:.3e}; mass drift {drift}")
     assert err <= 1e-5
-    assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-5
+    assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-8
+
+
+@pytest.mark.parametrize("N,steps", [(128, 20), (512, 8)])
+def test_headline_lattice_against_oracle(qx, N, steps):
+    """The headline workload's lattice (3D, L = 40, dt = 2.5e-4, GRF seed 0,
+    k = 0 zeroed) stepped on the GPU and by the FP64 oracle from the same
+    state: at 512^3 this runs the paths no smaller lattice reaches (16-column
+    tiles, streaming instantiations, the 512-point Poisson pass).  FP64 within
+    1e-12 and FP32 within 1e-5 relative L2 (FP32 measured 1.4e-6 after 50
+    steps against FP64); the 128-bin P(k) of |psi|^2 at the end per bin."""
+    L, dt = 40.0, 2.5e-4
+    print(f"[case] 3D {N}^3 oracle IC and {steps} steps", flush=True)
+    psi0 = o.ic_grf(3, N, L, 0)
+    V0, _, _ = o.update_potential(psi0, L, 1.0, 1.0, 0, 0)
+    want, Vw = o.leapfrog(psi0, V0, L, dt, steps, 1.0, 1.0, 0, 0)
+    nb = min(128, N // 2)
+    Po, co, _ = o.spectrum(np.abs(want) ** 2, L, nb, 2)
+    errs = {}
+    for prec in ("fp64", "fp32"):
+        print(f"[case] 3D {N}^3 {prec} {steps} steps", flush=True)
+        lat = qx.Lattice(3, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.psi = psi0
+        lat.update_potential()
+        lat.leapfrog(dt, steps)
+        P, c, _ = lat.spectrum(source="rho", nbins=nb, spb=2)
+        assert np.array_equal(c, co)
+        errs[prec] = (rel_l2(lat.psi, want), rel_l2(lat.V, Vw)) + spectrum_errors(P, Po, N, 2)
+        lat.close()
+    print(f"{N}^3 after {steps} steps (psi, V, P(k) in the cut, outside): {errs}")
+    e64, e32 = errs["fp64"], errs["fp32"]
+    assert e64[0] <= 1e-12 and e64[1] <= 1e-12 and e64[2] <= 1e-12
+    assert e32[0] <= 1e-5 and e32[1] <= 1e-5 and e32[2] <= 1e-5
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
