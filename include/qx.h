@@ -104,6 +104,21 @@ qx_status qx_compute_energy(qx_ctx *ctx, double *rhomax_over_mean);
 /* density statistics that leave V untouched; outputs nullable.  mass =
  * (L/N)^dim sum |psi|^2. */
 qx_status qx_density_stats(qx_ctx *ctx, double *rhomax_over_mean, double *mass);
+/* qx_density_stats without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  psi: a device array of S = N^dim complex
+ * values of the given precision (row-major, as a context's psi); the
+ * statistics go to out[0] = rho_max/mean and out[1] = mass = (L/N)^dim
+ * sum |psi|^2, a caller-owned device array of two doubles, equal bit for
+ * bit to qx_density_stats on the same psi.  Stream-ordered on `stream` (a
+ * hipStream_t; null = the default stream); returns without waiting, and
+ * allocates nothing: the reduction's partials go to a static 16 KiB device
+ * array (1024 pairs of doubles), one per GPU, held until the process ends.
+ * Lattices as qx_create accepts them; other sizes are rejected
+ * (QX_EINVAL).  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (different streams, or threads) race. */
+qx_status qx_density_stats_noalloc(const void *psi_dev, int dim, int N, double L, qx_precision prec,
+                                   void *stream, double *out_dev);
 /* mass, kinetic 1/2 dV/S sum k^2|psihat|^2 and potential 1/2 dV sum V|psi|^2
  * (no reference counterpart; outputs nullable) */
 qx_status qx_energy(qx_ctx *ctx, double *mass, double *kinetic, double *potential);

@@ -69,6 +69,8 @@ SIGNATURES = {
     "qx_poisson_half": (C.c_int, [C.c_void_p, _dp]),
     "qx_compute_energy": (C.c_int, [C.c_void_p, _dp]),
     "qx_density_stats": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "qx_density_stats_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                           C.c_void_p, C.c_void_p]),
     "qx_energy": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "qx_vmax": (C.c_double, [C.c_void_p]),
     "qx_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _i64p, _dp]),

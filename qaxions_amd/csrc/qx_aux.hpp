@@ -136,6 +136,22 @@ reduce_kernel(const typename cplx<R>::T *__restrict__ psi, const R *__restrict__
     }
 }
 
+// (sum, max) partials of reduce_kernel<., RED_RHO> summed in block order,
+// as the host sums them for qx_density_stats: out[0] = max/mean of |psi|^2,
+// out[1] = dV sum |psi|^2 (one thread: at most 1024 partials)
+__global__ void density_stats_finalize(const double *__restrict__ partials, int blocks, double sites,
+                                       double dV, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0, m = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        s += partials[2 * b];
+        m = fmax(m, partials[2 * b + 1]);
+    }
+    out[0] = m / (s / sites);
+    out[1] = dV * s;
+}
+
 // ---- power-spectrum binning of vt (src/spectrum/spectrum.cxx:63-155) -------
 // shell b = round(|n|) computed exactly in integers; bins gather `spb`
 // consecutive shells; Hermitian weight 1 for k_last in {0, N/2}, else 2.

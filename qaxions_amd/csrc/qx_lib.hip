@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -165,6 +166,7 @@ namespace {
 
 constexpr long long STAGE_ELEMS = 1ll << 22;  // complex128 elements per transfer chunk
 constexpr int MAX_PART = 1024;
+constexpr int MAX_DEVICES = 64;
 constexpr long long STREAM_IO_MIN_BYTES = 512ll << 20;
 
 qx_status ensure(qx_ctx *c, void **p, size_t bytes)
@@ -176,6 +178,38 @@ qx_status ensure(qx_ctx *c, void **p, size_t bytes)
         return fail(QX_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
     }
     (void)c;
+    return QX_OK;
+}
+
+}  // namespace
+
+// Device scratch of qx_density_stats_noalloc: the per-block (sum, max)
+// partials of its reduction, 2 * MAX_PART doubles (16 KiB).  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` names no device address: the copy of the current device is
+// reached through the address hipGetSymbolAddress gives for it
+// (scratch_address).
+static __device__ __align__(256) double scratch[2 * MAX_PART];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_addr[MAX_DEVICES] = {};
+
+// the device address of `scratch` on `dev`, looked up once per device with
+// that device current (the caller makes it current)
+qx_status scratch_address(int dev, double **out)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(QX_EINVAL, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(QX_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = (double *)p;
+    }
+    *out = g_scratch_addr[dev];
     return QX_OK;
 }
 
@@ -603,6 +637,8 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
         return cleanup(s);
     if (hipHostMalloc((void **)&c->h_partials, sizeof(double) * 2 * MAX_PART) != hipSuccess)
         return cleanup(fail(QX_ENOMEM, "pinned host allocation failed"));
+    double *scr = nullptr;  // qx_density_stats_noalloc's scratch on this device
+    if ((s = scratch_address(device, &scr)) != QX_OK) return cleanup(s);
 
     if (upload_twiddles(c->eng->tw, N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
     if (hipMemsetAsync(c->psi, 0, (size_t)c->S * c->csz, c->stream) != hipSuccess ||
@@ -874,6 +910,35 @@ qx_status qx_density_stats(qx_ctx *c, double *rhomax_over_mean, double *mass)
     TRY(reduce<RED_RHO>(c, c->psi, nullptr, c->S, &sum, &mx));
     if (rhomax_over_mean) *rhomax_over_mean = mx / (sum / (double)c->S);
     if (mass) *mass = std::pow(c->L / c->N, c->dim) * sum;
+    return QX_OK;
+}
+
+qx_status qx_density_stats_noalloc(const void *psi, int dim, int N, double L, qx_precision prec,
+                                   void *stream, double *out)
+{
+    if (!psi || !out) return fail(QX_EINVAL, "null psi or output");
+    if (dim != 2 && dim != 3) return fail(QX_EINVAL, "Dimension must be 2 or 3");
+    if (N < 2 || (N & (N - 1)) != 0 || N > (dim == 3 ? 1024 : 4096))
+        return fail(QX_EINVAL, "N must be a power of two in [2, 4096] (2D) or [2, 1024] (3D)");
+    if (!(L > 0.0)) return fail(QX_EINVAL, "L must be positive");
+    if (prec != QX_FP32 && prec != QX_FP64) return fail(QX_EINVAL, "unknown precision");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    double *scr = nullptr;
+    TRY(scratch_address(dev, &scr));
+    long long S = 1;
+    for (int d = 0; d < dim; ++d) S *= N;
+    // the grid of qx_density_stats: the same partials, summed in the same
+    // order, give the same bits
+    const unsigned blocks = grid_for(S, RED_THREADS, MAX_PART);
+    if (blocks > (unsigned)MAX_PART) return fail(QX_EINVAL, "more partials than the scratch holds");
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == QX_FP64)
+        reduce_kernel<double, RED_RHO><<<blocks, RED_THREADS, 0, st>>>((const double2 *)psi, nullptr, S, N, dim, scr);
+    else
+        reduce_kernel<float, RED_RHO><<<blocks, RED_THREADS, 0, st>>>((const float2 *)psi, nullptr, S, N, dim, scr);
+    density_stats_finalize<<<1, 1, 0, st>>>(scr, (int)blocks, (double)S, std::pow(L / N, dim), out);
+    HIPCHK(hipGetLastError());
     return QX_OK;
 }
 

@@ -56,3 +56,15 @@ def test_null_context_is_an_error(qx):
     assert lib.qx_kick(None, 0.1) == 1
     assert b"null" in lib.qx_last_error()
     assert lib.qx_destroy(None) == 0
+
+
+@pytest.mark.parametrize("dim,N,L,msg", [(4, 8, 1.0, "Dimension"), (3, 2048, 1.0, "power of two"),
+                                         (2, 12, 1.0, "power of two"), (2, 8, 0.0, "positive")])
+def test_noalloc_density_stats_rejects_bad_sizes(qx, dim, N, L, msg):
+    """qx_density_stats_noalloc checks its lattice on the host, before any
+    device call (no GPU here)."""
+    lib = qx.lib()
+    buf = ctypes.create_string_buffer(16)
+    assert lib.qx_density_stats_noalloc(ctypes.addressof(buf), dim, N, L, 0, None, ctypes.addressof(buf)) == 1
+    assert msg.encode() in lib.qx_last_error()
+    assert lib.qx_density_stats_noalloc(None, 2, 8, 1.0, 0, None, ctypes.addressof(buf)) == 1
