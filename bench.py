@@ -16,6 +16,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -69,19 +70,74 @@ def measured_traffic(pass_name, path=os.path.join(ROOT, "profiles", "kdk512_traf
         return None
 
 
-def cpu_baseline(lat, N, L, dt, steps, threads):
-    """FP64 oracle (oracle/qx_oracle.c, OpenMP) on the host: `steps` merged-kick
-    steps of the same lattice and IC, timed with a wall clock."""
-    import numpy as np
+def host_cpu():
+    """CPU model and physical core count of the host (lscpu), and the
+    threads the CPU leg may use: the physical cores, capped by the CPU share
+    this process is given (OMP_NUM_THREADS where set, else its affinity)."""
+    info = {"cpu_model": None, "physical_cores": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["cpu_model"] = kv.get("Model name")
+        cps, sockets = int(kv.get("Core(s) per socket", "0")), int(kv.get("Socket(s)", "0"))
+        if cps and sockets:
+            info["physical_cores"] = cps * sockets
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    share = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = min(share, int(os.environ["OMP_NUM_THREADS"]))
+    info["cpu_share"] = share
+    info["threads"] = min(share, info["physical_cores"] or share)
+    return info
+
+
+def spectrum_errors(P, Po, N, spb):
+    """max_b |P_b - Po_b| / Po_b over the bins whose shells lie inside the
+    GRF IC's spectral cut (|n| <= N/4)"""
+    cut = (np.arange(len(P)) + 1) * spb - 1 <= N // 4
+    return float((np.abs(P - Po)[cut] / Po[cut]).max())
+
+
+def oracle_leg(qx, N, L, dt, warm, timed, threads):
+    """The FP64 CPU oracle (oracle/qx_oracle.c, OpenMP, threads bound to
+    cores) on the headline lattice from the oracle's own GRF IC: `warm`
+    merged-kick steps, then `timed` steps each timed alone (their median is
+    the baseline).  The same steps from the same IC on the GPU, in FP32 and
+    FP64, are then compared with it: psi relative L2 and the per-bin P(k)."""
     import oracle
     oracle.set_threads(threads)
-    psi = lat.psi
-    V = lat.V
-    t0 = time.perf_counter()
-    oracle.leapfrog(psi, V, L, dt, steps, 1.0, 1.0, 0, 0)
-    t = time.perf_counter() - t0
-    del psi, V
-    return N ** 3 * steps / t, t, np
+    dim = 3
+    psi0 = oracle.ic_grf(dim, N, L, 0)
+    V, Vh, _ = oracle.update_potential(psi0, L, 1.0, 1.0, 0, 0)
+    psi = psi0.copy()
+    times = []
+    for i in range(warm + timed):
+        t0 = time.perf_counter()
+        oracle.leapfrog_inplace(psi, V, Vh, L, dt, 1, 1.0, 1.0, 0, 0, first_half=(i == 0), last_half=0)
+        if i >= warm:
+            times.append(time.perf_counter() - t0)
+    steps = warm + timed
+    nb = max(1, (N // 2) // 2)
+    Po, _, _ = oracle.spectrum(np.abs(psi) ** 2, L, nb, 2)
+    acc = {"steps": steps, "pk_bins": nb}
+    for prec in ("fp32", "fp64"):
+        lat = qx.Lattice(dim, N, L, precision=prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.psi = psi0
+        lat.update_potential()
+        lat.leapfrog(dt, steps, first_half=True, last_half=False)
+        got = lat.psi
+        acc[f"{prec}_rel_l2"] = float(np.linalg.norm(got - psi) / np.linalg.norm(psi))
+        P, _, _ = lat.spectrum(source="rho", nbins=nb, spb=2)
+        acc[f"{prec}_pk_max_rel_err_in_cut"] = spectrum_errors(P, Po, N, 2)
+        del got
+        lat.close()
+    med = float(np.median(times))
+    return N ** dim / med, med, times, acc
 
 
 def fp32_vs_fp64(qx, N, dim, L, dt, steps):
@@ -116,8 +172,9 @@ def main():
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--dim", type=int, default=3)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--cpu-steps", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="oracle steps timed one by one after the warm-up (median reported)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-against-FP64 run of the same workload")
@@ -137,6 +194,10 @@ def main():
               "with all-to-all transposes is designed, not built.", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU leg's OpenMP threads bound to cores (read when the oracle's
+    # library loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import qaxions_amd as qx
 
     if args.workload == "ensemble":
@@ -251,15 +312,24 @@ def main():
         print(f"[bench] FP32 vs FP64 psi rel L2 {acc['rel_l2']:.3e}; mass drift FP32 "
               f"{acc['mass_drift_fp32']:.2e}, FP64 {acc['mass_drift_fp64']:.2e}", file=sys.stderr, flush=True)
         out["fp32_vs_fp64"] = acc
-    if not args.no_cpu_baseline:
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline: {args.cpu_steps} steps of the FP64 oracle on {threads} threads",
-              file=sys.stderr, flush=True)
-        v, t, _ = cpu_baseline(lat, N, L, dt, args.cpu_steps, threads)
-        out["cpu_baseline"] = {"value": v, "unit": "site-updates/s", "cores": threads, "kind": "port",
-                               "sample": f"{args.cpu_steps} merged-kick KDK steps of the FP64 C oracle "
-                                         f"(radix-2 FFT, OpenMP) on the same {N}^{dim} lattice and IC, "
-                                         f"{t:.1f} s wall"}
+    if not args.no_cpu_baseline and dim == 3:
+        cpu = host_cpu()
+        print(f"[bench] cpu baseline: FP64 oracle, {args.cpu_warmup} + {args.cpu_steps} steps on "
+              f"{cpu['threads']} threads ({cpu['cpu_model']}, {cpu['physical_cores']} physical cores, "
+              f"share {cpu['cpu_share']})", file=sys.stderr, flush=True)
+        v, med, times, acc = oracle_leg(qx, N, L, dt, args.cpu_warmup, args.cpu_steps, cpu["threads"])
+        print(f"[bench] oracle {med:.3f} s/step (median of {len(times)}); after {acc['steps']} steps "
+              f"FP32 vs oracle {acc['fp32_rel_l2']:.3e}, FP64 vs oracle {acc['fp64_rel_l2']:.3e}, "
+              f"P(k) per bin FP32 {acc['fp32_pk_max_rel_err_in_cut']:.2e}", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = {"value": v, "unit": "site-updates/s", "cores": cpu["threads"], "kind": "port",
+                               "cpu_model": cpu["cpu_model"], "physical_cores": cpu["physical_cores"],
+                               "cpu_share": cpu["cpu_share"], "omp": "OMP_PROC_BIND=close OMP_PLACES=cores",
+                               "fft": "project radix-2 (FFTW absent)",
+                               "sample": f"median of {len(times)} merged-kick KDK steps, each timed alone after "
+                                         f"{args.cpu_warmup} warm-up steps, of the FP64 C oracle on the same "
+                                         f"{N}^{dim} lattice and IC",
+                               "step_s": times}
+        out["fp_vs_oracle"] = acc
     print(json.dumps(out), flush=True)
 
 

@@ -218,6 +218,19 @@ def leapfrog(psi, V, L, ds, nsteps, pref, rho_bar=1.0, k0_mode=1, rho_bar_mode=0
     return psi, V
 
 
+def leapfrog_inplace(psi, V, Vh, L, ds, nsteps, pref, rho_bar=1.0, k0_mode=1, rho_bar_mode=0,
+                     first_half=1, last_half=1):
+    """leapfrog on caller arrays, updated in place (no copies: timing of the
+    C loop alone); psi complex128, V float64 and Vh complex128 (half
+    spectrum), all C-contiguous"""
+    for a, t in ((psi, np.complex128), (V, np.float64), (Vh, np.complex128)):
+        if a.dtype != t or not a.flags.c_contiguous:
+            raise ValueError("leapfrog_inplace needs C-contiguous complex128/float64/complex128 arrays")
+    ph = Physics(pref, rho_bar, k0_mode, rho_bar_mode)
+    lib().qxo_leapfrog(psi.ndim, psi.shape[0], float(L), C.byref(ph), _p(psi.view(np.float64)), _p(V),
+                       _p(Vh.view(np.float64)), float(ds), int(nsteps), int(first_half), int(last_half))
+
+
 def propagate(psi, V, L, ds, pref, rho_bar=1.0, k0_mode=1, rho_bar_mode=0):
     psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
     V = np.ascontiguousarray(V, dtype=np.float64).copy()

@@ -36,3 +36,19 @@ def test_committed_traffic_record_covers_every_pass():
     assert bench.measured_traffic("c2r_kick_drift2", path) == pytest.approx(
         passes["c2r_kick_drift2"]["fetch_bytes"] + passes["c2r_kick_drift2"]["write_bytes"])
     assert bench.measured_traffic("no_such_pass", path) is None
+
+
+def test_cpu_leg_threads_follow_the_share():
+    info = bench.host_cpu()
+    assert 1 <= info["threads"] <= info["cpu_share"]
+    if info["physical_cores"]:
+        assert info["threads"] <= info["physical_cores"]
+
+
+def test_spectrum_errors_covers_the_bins_inside_the_cut():
+    import numpy as np
+    Po = np.array([1.0, 0.5, 0.25, 0.125, 1e-30])
+    P = Po * np.array([1.0, 1.0 + 1e-6, 1.0 - 2e-6, 1.0, 5.0])
+    # N = 16, two shells per bin: bins 0, 1 (shells up to 3) lie inside |n| <= 4
+    assert bench.spectrum_errors(P, Po, 16, 2) == pytest.approx(1e-6)
+    assert bench.spectrum_errors(P, Po, 32, 2) == pytest.approx(2e-6)
