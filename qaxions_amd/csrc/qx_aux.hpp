@@ -152,59 +152,60 @@ __global__ void density_stats_finalize(const double *__restrict__ partials, int 
     out[1] = dV * s;
 }
 
-// ---- power-spectrum binning of vt (src/spectrum/spectrum.cxx:63-155) -------
-// shell b = round(|n|) computed exactly in integers; bins gather `spb`
-// consecutive shells; Hermitian weight 1 for k_last in {0, N/2}, else 2.
-__device__ __forceinline__ int round_sqrt(long long m)
-{
-    long long s = (long long)sqrt((double)m);
-    while (s * s > m) --s;
-    while ((s + 1) * (s + 1) <= m) ++s;
-    return (int)((m > s * s + s) ? s + 1 : s);
-}
+// ---- power-spectrum binning (src/spectrum/spectrum.cxx:63-155) -----------
+// shell = round(|n|) (shell_of), bins of `spb` consecutive shells; the power
+// itself is binned inside the transform's last pass (strided_bin,
+// contig_bin)
 
-template <typename R>
+// mode counts per bin (the data-independent half of the binning): every
+// half-spectrum mode adds its weight w (1 for k_last in {0, N/2}, else 2)
+// to the bin of its shell
 __global__ void __launch_bounds__(256)
-spectrum_kernel(const typename cplx<R>::T *__restrict__ vt, long long rows, int N, int dim,
-                int nbins, int spb, double norm, double *__restrict__ P,
-                unsigned long long *__restrict__ cnt)
+spectrum_count_kernel(long long rows, int N, int dim, int nbins, int spb, unsigned long long *__restrict__ cnt)
 {
-    extern __shared__ double shm[];
-    double *sp = shm;
-    unsigned long long *sc = (unsigned long long *)(shm + nbins);
-    for (int i = threadIdx.x; i < nbins; i += blockDim.x) { sp[i] = 0.0; sc[i] = 0ull; }
+    extern __shared__ unsigned int sc[];
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) sc[i] = 0u;
     __syncthreads();
-    const int H1 = N / 2 + 1, nshell = N / 2 + 1;
+    const int H1 = N / 2 + 1;
     const long long total = (long long)H1 * rows;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int k = (int)(i / rows);
         const long long row = i % rows;
-        long long m;
+        int m = k * k;
         if (dim == 3) {
             const int n0 = foldi((int)(row / N), N), n1 = foldi((int)(row % N), N);
-            m = (long long)n0 * n0 + (long long)n1 * n1 + (long long)k * k;
+            m += n0 * n0 + n1 * n1;
         } else {
             const int n0 = foldi((int)row, N);
-            m = (long long)n0 * n0 + (long long)k * k;
+            m += n0 * n0;
         }
-        const int shell = round_sqrt(m);
-        if (shell >= nshell) continue;
-        const int b = shell / spb;
-        if (b >= nbins) continue;
-        const int w = (k == 0 || k == N / 2) ? 1 : 2;
-        auto x = vt[i];
-        const double p = (double)w * norm * ((double)x.x * x.x + (double)x.y * x.y);
-        atomicAdd(&sp[b], p);
-        atomicAdd(&sc[b], (unsigned long long)w);
+        const int shell = shell_of(m);
+        if (shell > N / 2 || shell / spb >= nbins) continue;
+        atomicAdd(&sc[shell / spb], (k == 0 || k == N / 2) ? 1u : 2u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-        if (sc[i]) {
-            atomicAdd(&P[i], sp[i]);
-            atomicAdd(&cnt[i], sc[i]);
-        }
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+        if (sc[i]) atomicAdd(&cnt[i], (unsigned long long)sc[i]);
+}
+
+// out[bin] = sum over the rows r < rows of part[r][bin] (the binning
+// pass's block histograms): one block per bin, each thread a strided set
+// of rows in order, then a fixed tree, so the sum does not depend on timing
+__global__ void __launch_bounds__(256)
+bin_reduce(const double *__restrict__ part, int rows, int nbins, double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    const int bin = blockIdx.x;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += part[(long long)r * nbins + bin];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[bin] = sh[0];
 }
 
 // ---- vt[k*rows + row] <-> ref[row*H1 + k] (the reference's half layout) ------

@@ -34,6 +34,11 @@ struct Engine {
                           unsigned long long *vmax, hipStream_t s, const void *tiles = nullptr,
                           const double2 *ptab = nullptr, const float2 *ftab = nullptr,
                           long long ptab_stride = 0, double inv_n = 0.0) = 0;
+    // P(k) binned inside the last forward pass of the spectrum's transform
+    // (3D: the i0 pass over vt [k2][i0][k1]; 2D: the i0 lines of vt
+    // [k1][i0]); `units` of work (column tiles or line blocks) for the grid
+    virtual long long bin_units(int dim) const = 0;
+    virtual void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) = 0;
     void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N (FP32: then their low words, see cmul2)
     // streaming loads and stores in the passes (lattices whose arrays are far
     // larger than the memory-side cache; instantiated for N >= 512 only)
@@ -148,6 +153,19 @@ template <int N, typename R> struct EngineT final : Engine {
         if constexpr (NT_OK)
             if (stream_io) return r2c_rows_t<true>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
         r2c_rows_t<false>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
+    }
+    long long bin_units(int dim) const override
+    {
+        return dim == 3 ? (long long)(N / 2 + 1) * (N / (G::TC < N ? G::TC : N)) : (N / 2 + G::LPB) / G::LPB;
+    }
+    void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) override
+    {
+        if (dim == 3) {
+            if constexpr (N <= 1024)
+                strided_bin<N, R><<<grid, (G::TC < N ? G::TC : N) * P::TPL, 0, s>>>((const C *)vt, ba, (const C *)tw);
+        } else {
+            contig_bin<N, R><<<grid, G::CONTIG_THREADS, 0, s>>>((const C *)vt, ba, (const C *)tw);
+        }
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
                   const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s,

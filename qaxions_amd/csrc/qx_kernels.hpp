@@ -530,6 +530,172 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
 }
 
 // ---------------------------------------------------------------------------
+// Power spectrum (src/spectrum/spectrum.cxx:63-155) binned inside the last
+// forward pass of the spectrum's transform, whose output is then never
+// written: each mode adds w |X|^2 (w = 1 for k_last in {0, N/2}, else 2) to
+// the bin of its integer shell round(|n|).  After the transform one more LDS
+// exchange gives each thread E consecutive frequencies of its line; along
+// them |n| changes slowly, so the thread sums runs of equal bins in a
+// register and adds each run to the block's FP64 histogram in LDS with one
+// atomic (a flush whenever the bin changes, correct in any order).  Each
+// block writes its histogram to partials[block][nbins], which bin_reduce
+// sums in a fixed order.
+// ---------------------------------------------------------------------------
+struct BinArgs {
+    int nbins, spb;          // bin j gathers the shells [j spb, (j+1) spb)
+    double *partials;        // [gridDim.x][nbins]
+};
+
+// round(sqrt(m)) for 0 <= m < 2^24, in FP32 where m, s^2, s^2 + s and
+// (s + 1)^2 are all exact: the hardware square root is within one of the
+// integer square root s, which one step each way makes exact; round up
+// where m > s^2 + s, i.e. m > (s + 1/2)^2
+__device__ __forceinline__ int shell_of(int m)
+{
+    const float mf = (float)m;
+    float s = floorf(__builtin_amdgcn_sqrtf(mf));
+    s = (s * s > mf) ? s - 1.0f : s;
+    s = ((s + 1.0f) * (s + 1.0f) <= mf) ? s + 1.0f : s;
+    return (int)((mf > fmaf(s, s, s)) ? s + 1.0f : s);
+}
+
+// the block's histogram and the bin of each shell (entries up to shell
+// N/2 + 1; shells beyond N/2, and beyond the last bin, map to -1)
+template <int N> __device__ __forceinline__ void bin_tables(double *hist, short *sbin, const BinArgs &ba)
+{
+    for (int i = threadIdx.x; i < ba.nbins; i += blockDim.x) hist[i] = 0.0;
+    for (int s = threadIdx.x; s <= N / 2 + 1; s += blockDim.x)
+        sbin[s] = (short)((s <= N / 2 && s / ba.spb < ba.nbins) ? s / ba.spb : -1);
+}
+// next shell along a line: |n| changes by at most one per step, so the
+// shell s of the previous position moves by at most one (FP32 exact, as in
+// shell_of)
+__device__ __forceinline__ float shell_step(float s, int m)
+{
+    const float mf = (float)m;
+    s = (mf > fmaf(s, s, s)) ? s + 1.0f : s;
+    return (mf <= fmaf(s, s, -s) && s > 0.0f) ? s - 1.0f : s;
+}
+
+// this thread's E consecutive frequencies p0 .. p0 + E - 1 of one line
+// (m0 = the squared frequencies of the other axes, w the Hermitian weight)
+template <int N, int E, typename C>
+__device__ __forceinline__ void bin_run(double *hist, const short *sbin, const C (&x)[E], int p0, int m0, double w)
+{
+    int cur = -1;
+    double acc = 0.0;
+    float sh = 0.0f;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const int n = foldi(p0 + j, N);
+        sh = j == 0 ? (float)shell_of(n * n + m0) : shell_step(sh, n * n + m0);
+        const int bin = sbin[min((int)sh, N / 2 + 1)];
+        if (bin != cur) {
+            if (cur >= 0) atomicAdd(&hist[cur], acc);
+            cur = bin;
+            acc = 0.0;
+        }
+        acc = fma(w, fma((double)x[j].x, (double)x[j].x, (double)x[j].y * (double)x[j].y), acc);
+    }
+    if (cur >= 0) atomicAdd(&hist[cur], acc);
+}
+template <int N>
+__device__ __forceinline__ void bin_flush(const double *hist, const BinArgs &ba)
+{
+    __syncthreads();
+    for (int i = threadIdx.x; i < ba.nbins; i += blockDim.x) ba.partials[(long long)blockIdx.x * ba.nbins + i] = hist[i];
+}
+
+// 3D: vt [k2][i0][k1] after the i1 transform; columns along i0 (stride N),
+// a block's tiles of TC columns of one k2 plane in turn (grid-stride over
+// the H1 * N/TC tiles); the i0 transform then the binning
+// waves per SIMD that two blocks of strided_bin make (the LDS holds two):
+// the hint keeps the registers within that occupancy (N = 1024)
+template <int N, typename R> struct BinWaves {
+    static constexpr int TC = Geom<N, R>::TC < N ? Geom<N, R>::TC : N;
+    static constexpr int W = 2 * (TC * Plan<N>::TPL / 64) / 4;
+    static constexpr int value = N == 1024 ? (W > 8 ? 8 : (W < 1 ? 1 : W)) : 1;
+};
+
+template <int N, typename R>
+__global__ void __launch_bounds__(((Geom<N, R>::TC < N ? Geom<N, R>::TC : N) * Plan<N>::TPL))
+__attribute__((amdgpu_waves_per_eu(BinWaves<N, R>::value)))
+strided_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = Plan<N>;
+    // columns per block: the strided width, at most the plane's N columns
+    constexpr int TC = Geom<N, R>::TC < N ? Geom<N, R>::TC : N;
+    __shared__ C lds[N * TC];
+    __shared__ C tw[N];
+    __shared__ double hist[N / 2 + 1];
+    __shared__ short sbin[N / 2 + 2];
+    load_tw<N>(tw, twg);
+    bin_tables<N>(hist, sbin, ba);
+    __syncthreads();
+    const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+    constexpr int TPO = N / TC;
+    constexpr int TILES = (N / 2 + 1) * TPO;
+    for (int b = blockIdx.x; b < TILES; b += gridDim.x) {
+        const int k2 = b / TPO;
+        const int k1 = (b % TPO) * TC + c;
+        const C *src = data + (long long)k2 * N * N + k1;
+        C v[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = src[(t + q * P::TPL) * N];
+        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
+        // positions E t .. E t + E - 1 of this column
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) lds[(t + q * P::TPL) * TC + c] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < P::E; ++j) v[j] = lds[(P::E * t + j) * TC + c];
+        __syncthreads();
+        const int n1 = foldi(k1, N);
+        bin_run<N, P::E>(hist, sbin, v, P::E * t, n1 * n1 + k2 * k2, (k2 == 0 || k2 == N / 2) ? 1.0 : 2.0);
+    }
+    bin_flush<N>(hist, ba);
+}
+
+// 2D: vt [k1][i0]; lines along i0, one per k1 (grid-stride over blocks of
+// LPB lines); the i0 transform then the binning
+template <int N, typename R>
+__global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
+contig_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const typename cplx<R>::T *__restrict__ twg)
+{
+    using C = typename cplx<R>::T;
+    using P = typename Geom<N, R>::PR;
+    using G_ = Geom<N, R>;
+    __shared__ C lds[G_::LPB * P::PADDED];
+    __shared__ C tw[N];
+    __shared__ double hist[N / 2 + 1];
+    __shared__ short sbin[N / 2 + 2];
+    load_tw<N>(tw, twg);
+    bin_tables<N>(hist, sbin, ba);
+    __syncthreads();
+    const int t = threadIdx.x % P::TPL, l = threadIdx.x / P::TPL;
+    C *mine = lds + l * P::PADDED;
+    constexpr int LINES = N / 2 + 1;
+    for (int lb = blockIdx.x; lb * G_::LPB < LINES; lb += gridDim.x) {
+        const int line = lb * G_::LPB + l;
+        const bool ok = line < LINES;
+        C v[P::E];
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = ok ? data[(long long)line * N + t + q * P::TPL] : C{0, 0};
+        fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
+        // positions E t .. E t + E - 1 of this line
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = v[q];
+        line_sync<N, LdsContig, P::E>();
+#pragma unroll
+        for (int j = 0; j < P::E; ++j) v[j] = mine[LdsContig::at(P::E * t + j, 1)];
+        line_sync<N, LdsContig, P::E>();
+        if (ok) bin_run<N, P::E>(hist, sbin, v, P::E * t, line * line, (line == 0 || line == N / 2) ? 1.0 : 2.0);
+    }
+    bin_flush<N>(hist, ba);
+}
+
+// ---------------------------------------------------------------------------
 // real rows -> vt (half spectrum, k-major), two rows per complex transform
 // ---------------------------------------------------------------------------
 enum {

@@ -138,9 +138,13 @@ struct qx_ctx {
     double *partials = nullptr;      // device reduction partials
     double *h_partials = nullptr;    // pinned host copy
     unsigned long long *vmax_bits = nullptr;
-    double *spec_P = nullptr;
+    // P(k): block histograms (up to SPEC_BLOCKS rows of N/2+1 bins) and the
+    // final bins; mode counts per bin, computed once per binning (nbins,
+    // spb) and kept
+    double *spec_part = nullptr, *spec_P = nullptr;
     unsigned long long *spec_cnt = nullptr;
-    int spec_cap = 0;
+    std::vector<int64_t> spec_counts;
+    int spec_key_nbins = 0, spec_key_spb = 0;
     hipStream_t stream = nullptr;
     qx_physics ph{4.0 * M_PI, 1.0, 1, 0};
     double vmax = 0.0;
@@ -167,6 +171,7 @@ namespace {
 constexpr long long STAGE_ELEMS = 1ll << 22;  // complex128 elements per transfer chunk
 constexpr int MAX_PART = 1024;
 constexpr int MAX_DEVICES = 64;
+constexpr int SPEC_BLOCKS = 2048;   // blocks of the binning pass (grid-stride)
 constexpr long long STREAM_IO_MIN_BYTES = 512ll << 20;
 
 qx_status ensure(qx_ctx *c, void **p, size_t bytes)
@@ -656,7 +661,7 @@ qx_status qx_destroy(qx_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->psi, c->V, c->vt, c->psit, c->vtk, c->aux_c, c->aux_vt, c->aux_r, c->stage,
-                    c->partials, c->vmax_bits, c->spec_P, c->spec_cnt, c->ptab, c->sched,
+                    c->partials, c->vmax_bits, c->spec_part, c->spec_P, c->spec_cnt, c->ptab, c->sched,
                     c->eng ? c->eng->tw : nullptr};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -975,39 +980,44 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
     if (nbins < 1 || spb < 1 || nbins > c->N / 2 + 1)
         return fail(QX_EINVAL, "nbins must be in [1, N/2+1] and spb >= 1");
     if (!Pk || !count || !kmodes) return fail(QX_EINVAL, "null output");
+    const int nsh = c->N / 2 + 1;
     TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
-    if (c->spec_cap < nbins) {
-        if (c->spec_P) (void)hipFree(c->spec_P);
-        if (c->spec_cnt) (void)hipFree(c->spec_cnt);
-        c->spec_P = nullptr;
-        c->spec_cnt = nullptr;
-        c->spec_cap = 0;
-        TRY(ensure(c, (void **)&c->spec_P, sizeof(double) * (c->N / 2 + 1)));
-        TRY(ensure(c, (void **)&c->spec_cnt, sizeof(unsigned long long) * (c->N / 2 + 1)));
-        c->spec_cap = c->N / 2 + 1;
-    }
+    TRY(ensure(c, (void **)&c->spec_part, sizeof(double) * SPEC_BLOCKS * nsh));
+    TRY(ensure(c, (void **)&c->spec_P, sizeof(double) * nsh));
+    TRY(ensure(c, (void **)&c->spec_cnt, sizeof(unsigned long long) * nsh));
+    // the field's half spectrum: r2c along the last axis (of V, or of |psi|^2
+    // formed on the fly), then every other axis but the first; the first
+    // axis' transform bins the modes and writes nothing
     if (source == 0)
         c->eng->r2c_rows(R2C_REAL, nullptr, c->V, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     else
         c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
-    vt_forward(c, c->aux_vt);
-    HIPCHK(hipMemsetAsync(c->spec_P, 0, sizeof(double) * nbins, c->stream));
-    HIPCHK(hipMemsetAsync(c->spec_cnt, 0, sizeof(unsigned long long) * nbins, c->stream));
-    const double norm = 1.0 / ((double)c->S * (double)c->S);
-    const size_t shm = (size_t)nbins * (sizeof(double) + sizeof(unsigned long long));
-    const unsigned grid = grid_for(c->H, 256, 2048);
-    if (fp64(c))
-        spectrum_kernel<double><<<grid, 256, shm, c->stream>>>((const double2 *)c->aux_vt, c->rows, c->N, c->dim, nbins, spb, norm, c->spec_P, c->spec_cnt);
-    else
-        spectrum_kernel<float><<<grid, 256, shm, c->stream>>>((const float2 *)c->aux_vt, c->rows, c->N, c->dim, nbins, spb, norm, c->spec_P, c->spec_cnt);
-    std::vector<unsigned long long> cnt(nbins);
+    MulArgs none{};
+    if (c->dim == 3) c->eng->contig(c->aux_vt, c->H1 * c->N, -1, none, c->stream);
+    const unsigned grid = (unsigned)std::min<long long>(SPEC_BLOCKS, c->eng->bin_units(c->dim));
+    const BinArgs ba{nbins, spb, c->spec_part};
+    c->eng->spectrum_bin(c->dim, c->aux_vt, ba, grid, c->stream);
+    bin_reduce<<<nbins, 256, 0, c->stream>>>(c->spec_part, (int)grid, nbins, c->spec_P);
     HIPCHK(hipMemcpyAsync(Pk, c->spec_P, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(cnt.data(), c->spec_cnt, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost, c->stream));
+    if (c->spec_key_nbins != nbins || c->spec_key_spb != spb) {
+        // counts of this binning: once, then kept
+        std::vector<unsigned long long> cnt(nbins);
+        HIPCHK(hipMemsetAsync(c->spec_cnt, 0, sizeof(unsigned long long) * nbins, c->stream));
+        spectrum_count_kernel<<<grid_for(c->H, 256, 2048), 256, sizeof(unsigned int) * nbins, c->stream>>>(
+            c->rows, c->N, c->dim, nbins, spb, c->spec_cnt);
+        HIPCHK(hipMemcpyAsync(cnt.data(), c->spec_cnt, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost,
+                              c->stream));
+        TRY(sync(c));
+        c->spec_counts.assign(cnt.begin(), cnt.end());
+        c->spec_key_nbins = nbins;
+        c->spec_key_spb = spb;
+    }
     TRY(sync(c));
+    const double norm = 1.0 / ((double)c->S * (double)c->S);
     const double dk = TWO_PI / c->L;
     for (int b = 0; b < nbins; ++b) {
-        count[b] = (int64_t)cnt[b];
-        if (cnt[b] > 0) Pk[b] /= (double)cnt[b];
+        count[b] = c->spec_counts[b];
+        Pk[b] = count[b] > 0 ? Pk[b] * norm / (double)count[b] : 0.0;
         kmodes[b] = ((double)b * spb + 0.5 * (spb - 1)) * dk;
     }
     return QX_OK;

@@ -137,7 +137,9 @@ def test_energy(qx, dim, N, L, prec):
 
 
 @pytest.mark.parametrize("dim,N,nbins,spb", [(2, 16, 9, 1), (2, 256, 64, 2), (3, 16, 9, 1),
-                                             (3, 32, 8, 2), (2, 10, 6, 1)])
+                                             (3, 32, 8, 2), (2, 10, 6, 1), (2, 4, 3, 1), (3, 4, 3, 1),
+                                             (3, 64, 33, 1), (3, 128, 21, 3), (2, 1024, 100, 5),
+                                             (2, 4096, 2049, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_spectrum(qx, dim, N, nbins, spb, prec):
     if N & (N - 1):
@@ -153,9 +155,11 @@ def test_spectrum(qx, dim, N, nbins, spb, prec):
     assert np.array_equal(c, co)
     assert np.allclose(k, ko, rtol=1e-15)
     # bins inside the IC's spectral cut: per-bin relative error (FP32
-    # measured <= 1.4e-7 at 512^3); the bins beyond it hold round-off only
-    # (|psi|^2 of this IC is band-limited), compared against bin 0
-    rtol, otol = (1e-12, 1e-20) if prec == "fp64" else (1e-6, 1e-12)
+    # measured <= 1.4e-7 at 512^3, 1.8e-6 at 2D 4096^2 in single-shell bins:
+    # the round-off of log2(S) stages against the mean's share of |rho|^2);
+    # the bins beyond it hold round-off only (|psi|^2 of this IC is
+    # band-limited), compared against bin 0
+    rtol, otol = (1e-12, 1e-20) if prec == "fp64" else (max(1e-6, 1e-7 * math.log2(N ** dim)), 1e-12)
     rel, out = spectrum_errors(P, Po, N, spb)
     print(f"P(k) {prec}: max rel err in the cut {rel:.2e}, outside {out:.2e}")
     assert rel <= rtol and out <= otol
