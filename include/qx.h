@@ -270,13 +270,20 @@ qx_status qx_net_train_step_lattices(qx_net *net, qx_ctx *coarse, qx_ctx *fine, 
 /* mean((x - target)^2) over two psi arrays [H*W][2]: the loss an identity
  * corrector would have */
 qx_status qx_net_pair_mse(qx_net *net, const void *x_dev, const void *t_dev, double *mse);
-/* psi <- psi + s net(psi, a V) on a 2D FP32 context of size H x W (V untouched) */
+/* psi <- psi + s net(psi, a V) on a 2D FP32 context of size H x W, then
+ * scaled back to the mass it had before (V untouched) */
 qx_status qx_net_correct(qx_net *net, qx_ctx *ctx);
+/* energy gate of qx_net_coarse_steps: a chunk whose relative change of the
+ * energy (kinetic + potential, qx_energy) exceeds tol is redone with fine
+ * steps, as one beyond mass_tol is; 0 (the default) turns the gate off */
+qx_status qx_net_set_energy_tol(qx_net *net, double tol);
 /* nsteps corrected coarse steps of k*dt each (leapfrog(k dt, 1), correct,
- * update the potential), stream-ordered with no host waits inside a chunk
- * of check_every steps; after each chunk the relative mass change is read
- * back and, beyond mass_tol, the chunk is redone as k*chunk fine steps from
- * the saved state (*fallbacks = number of redone chunks, nullable) */
+ * scale psi back to its mass at the start of the call, update the
+ * potential), stream-ordered with no host waits inside a chunk of
+ * check_every steps; after each chunk the relative mass change (and the
+ * energy's, with an energy tolerance set) is read back and, beyond its
+ * tolerance, the chunk is redone as k*chunk fine steps from the saved
+ * state (*fallbacks = number of redone chunks, nullable) */
 qx_status qx_net_coarse_steps(qx_net *net, qx_ctx *ctx, double dt, int k, int nsteps, int check_every,
                               double mass_tol, int *fallbacks);
 void *qx_net_stream(qx_net *net);

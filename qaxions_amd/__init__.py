@@ -128,6 +128,7 @@ SIGNATURES = {
     "qx_net_pair_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     "qx_net_coarse_steps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
                                       C.c_double, C.POINTER(C.c_int)]),
+    "qx_net_set_energy_tol": (C.c_int, [C.c_void_p, C.c_double]),
     "qx_net_stream": (C.c_void_p, [C.c_void_p]),
     "qx_net_last_error": (C.c_char_p, []),
     "qx_tune": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int)]),
@@ -642,6 +643,11 @@ class Corrector:
 
     def correct(self, lat):
         _check_net(lib().qx_net_correct(self._h, lat.handle))
+
+    def set_energy_tol(self, tol):
+        """coarse_steps' energy gate: chunks whose relative energy change
+        exceeds tol fall back to fine steps (0 turns it off)"""
+        _check_net(lib().qx_net_set_energy_tol(self._h, float(tol)))
 
     def coarse_steps(self, lat, dt, k, nsteps, check_every=10, mass_tol=1e-6):
         """nsteps corrected coarse steps of k*dt; returns the number of chunks

@@ -421,6 +421,40 @@ __global__ void input_kernel(const float *__restrict__ psi, const float *__restr
     }
 }
 
+// sum |y|^2 of P complex values: FP64 per-block partials (grid <= 1024)
+__global__ void __launch_bounds__(256)
+norm2_partial(const float *__restrict__ y, long long P, double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < P;
+         p += (long long)gridDim.x * blockDim.x)
+        s += (double)y[2 * p] * y[2 * p] + (double)y[2 * p + 1] * y[2 * p + 1];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+// psi = y sqrt(target / sum of the partials): the corrected state scaled
+// back to the mass it had before the correction (partials summed in order
+// by every thread, an FP64 sum of at most 1024 terms)
+__global__ void __launch_bounds__(256)
+renorm_copy(const float *__restrict__ y, const double *__restrict__ partial, int nb, double target,
+            long long n, float *__restrict__ psi)
+{
+    __shared__ float scale;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += partial[b];
+        scale = (float)sqrt(target / s);
+    }
+    __syncthreads();
+    const float f = scale;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        psi[i] = y[i] * f;
+}
+
 unsigned grid1(long long n, int t = 256)
 {
     long long b = (n + t - 1) / t;
@@ -450,6 +484,7 @@ struct qx_net {
     bool packed = false;
     int wchunk = 512, wblocks = 0;
     void *save_psi = nullptr, *save_V = nullptr;  // fallback copies (lazy)
+    double energy_tol = 0.0;  // coarse_steps: energy-drift gate per chunk (0: off)
 };
 
 namespace {
@@ -768,24 +803,60 @@ static void lattice_input(qx_net *n, const float *psi, const float *V, hipStream
     input_kernel<<<grid1(n->P), 256, 0, s>>>(psi, V, n->in_scale, n->P, n->xin);
 }
 
-// psi <- psi + net(psi, V) on a 2D FP32 lattice; V is not updated
+// the corrected state y scaled to sum |psi|^2 = target, written to psi
+static qx_status renormalize_into(qx_net *n, float *psi, double target, hipStream_t s)
+{
+    const unsigned nb = std::min<unsigned>(grid1(n->P), 1024);
+    norm2_partial<<<nb, 256, 0, s>>>(n->y, n->P, n->lpart);
+    renorm_copy<<<grid1(2 * n->P), 256, 0, s>>>(n->y, n->lpart, (int)nb, target, 2 * n->P, psi);
+    NCHK(hipGetLastError());
+    return QX_OK;
+}
+
+// sum |psi|^2 of a device psi array (FP64, partials summed on the host)
+static qx_status norm2_host(qx_net *n, const float *psi, hipStream_t s, double *out)
+{
+    const unsigned nb = std::min<unsigned>(grid1(n->P), 1024);
+    norm2_partial<<<nb, 256, 0, s>>>(psi, n->P, n->lpart);
+    NCHK(hipMemcpyAsync(n->h_lpart, n->lpart, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    NCHK(hipStreamSynchronize(s));
+    double t = 0.0;
+    for (unsigned b = 0; b < nb; ++b) t += n->h_lpart[b];
+    *out = t;
+    return QX_OK;
+}
+
+// psi <- psi + net(psi, V) on a 2D FP32 lattice, scaled back to its mass
+// before the correction; V is not updated
 qx_status qx_net_correct(qx_net *n, qx_ctx *ctx)
 {
     float *psi, *V;
     NTRY(check_ctx(n, ctx, &psi, &V));
     NCHK(hipSetDevice(n->device));
     NCHK(hipStreamSynchronize((hipStream_t)qx_stream(ctx)));
+    double target = 0.0;
+    NTRY(norm2_host(n, psi, n->stream, &target));
     lattice_input(n, psi, V, n->stream);
     NTRY(forward(n, n->xin, n->y, false));
-    NCHK(hipMemcpyAsync(psi, n->y, sizeof(float) * n->P * 2, hipMemcpyDeviceToDevice, n->stream));
+    NTRY(renormalize_into(n, psi, target, n->stream));
     return nsync(n);
 }
 
+qx_status qx_net_set_energy_tol(qx_net *n, double tol)
+{
+    if (!n || !(tol >= 0.0)) return nfail(QX_EINVAL, "energy tolerance must be >= 0");
+    n->energy_tol = tol;
+    return QX_OK;
+}
+
 // nsteps corrected coarse steps of length k*dt, each: leapfrog(k dt, 1),
-// psi += net(psi), potential update, all stream-ordered on the lattice's
-// stream.  Every `check_every` steps the relative mass change over the chunk
-// is read back; beyond mass_tol the chunk is redone as k*chunk fine steps
-// from the saved state (*fallbacks counts redone chunks).
+// psi += net(psi), psi scaled back to the mass it had when the call began
+// (the steps are unitary, so that is its mass), potential update; all
+// stream-ordered on the lattice's stream.  Every `check_every` steps the
+// relative mass change over the chunk, and with an energy tolerance set
+// (qx_net_set_energy_tol) the relative change of the energy, are read back;
+// beyond either tolerance the chunk is redone as k*chunk fine steps from
+// the saved state (*fallbacks counts redone chunks).
 qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nsteps, int check_every,
                               double mass_tol, int *fallbacks)
 {
@@ -799,8 +870,17 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
     if (!n->save_V && hipMalloc(&n->save_V, bv) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
     hipStream_t s = (hipStream_t)qx_stream(ctx);
     int fb = 0;
-    double m0 = 0.0;
+    double m0 = 0.0, target = 0.0, e0 = 0.0;
+    const bool egate = n->energy_tol > 0.0;
+    auto energy = [&](double *e) -> qx_status {
+        double m, kin, pot;
+        if (qx_energy(ctx, &m, &kin, &pot) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
+        *e = kin + pot;
+        return QX_OK;
+    };
     if (qx_density_stats(ctx, nullptr, &m0) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
+    NTRY(norm2_host(n, psi, s, &target));
+    if (egate) NTRY(energy(&e0));
     for (int done = 0; done < nsteps;) {
         const int chunk = std::min(check_every, nsteps - done);
         NCHK(hipMemcpyAsync(n->save_psi, psi, bp, hipMemcpyDeviceToDevice, s));
@@ -809,19 +889,24 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
             if (qx::leapfrog_enqueue(ctx, k * dt, 1, 1, 1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
             lattice_input(n, psi, V, s);
             NTRY(forward(n, n->xin, n->y, false, s));
-            NCHK(hipMemcpyAsync(psi, n->y, bp, hipMemcpyDeviceToDevice, s));
+            NTRY(renormalize_into(n, psi, target, s));
             if (qx::update_potential_enqueue(ctx) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
         }
-        double m1 = 0.0;
+        double m1 = 0.0, e1 = 0.0;
         if (qx_density_stats(ctx, nullptr, &m1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
-        if (!(std::fabs(m1 - m0) <= mass_tol * m0)) {
+        if (egate) NTRY(energy(&e1));
+        const bool bad = !(std::fabs(m1 - m0) <= mass_tol * m0) ||
+                         (egate && !(std::fabs(e1 - e0) <= n->energy_tol * std::fabs(e0)));
+        if (bad) {
             ++fb;
             NCHK(hipMemcpyAsync(psi, n->save_psi, bp, hipMemcpyDeviceToDevice, s));
             NCHK(hipMemcpyAsync(V, n->save_V, bv, hipMemcpyDeviceToDevice, s));
             if (qx_leapfrog(ctx, dt, k * chunk, 1, 1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
             if (qx_density_stats(ctx, nullptr, &m1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
+            if (egate) NTRY(energy(&e1));
         }
         m0 = m1;
+        e0 = e1;
         done += chunk;
     }
     if (fallbacks) *fallbacks = fb;

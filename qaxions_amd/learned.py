@@ -117,5 +117,26 @@ def evaluate(net, N, L, dt, k, seed, n_coarse, check_every=10, mass_tol=1e-6, re
     out["learned"]["rel_l2_vs_fine"] = _rel(runs["learned"][0], runs["fine"][0])
     out["coarse"]["rel_l2_vs_fine"] = _rel(runs["coarse"][0], runs["fine"][0])
     out["speedup_learned_vs_fine"] = runs["fine"][1] / runs["learned"][1]
+    out.update(equal_error(out, dt, k))
+    out["horizon"] = {"n_coarse": n_coarse, "time": n_coarse * k * dt}
     lat0.close()
     return out
+
+
+def equal_error(res, dt, k):
+    """Speed-up of corrected coarse stepping over plain stepping at the same
+    error.  Plain KDK stepping's error against the reference grows as h^2:
+    e(h) = C h^2, with C fitted to the plain fine (h = dt) and coarse
+    (h = k dt) errors.  Plain stepping reaches the corrected run's error at
+    h_eq = sqrt(e_learned / C), at a cost of (fine run's time) * dt / h_eq;
+    the speed-up is that cost over the corrected run's time.  The gain the
+    network itself brings is e_coarse / e_learned (1: none)."""
+    ef, ec = res["fine"]["rel_l2_vs_reference"], res["coarse"]["rel_l2_vs_reference"]
+    el = res["learned"]["rel_l2_vs_reference"]
+    C = (ef / dt ** 2 + ec / (k * dt) ** 2) / 2.0
+    h_eq = float(np.sqrt(el / C)) if C > 0 else float("nan")
+    t_plain = res["fine"]["seconds"] * dt / h_eq
+    return {"error_gain_learned_vs_coarse": ec / el if el > 0 else float("inf"),
+            "order_check_coarse_over_fine": ec / ef if ef > 0 else float("nan"),
+            "h_equal_error": h_eq,
+            "speedup_learned_vs_plain_at_equal_error": t_plain / res["learned"]["seconds"]}

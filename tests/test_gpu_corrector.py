@@ -128,3 +128,69 @@ def test_zero_network_is_plain_coarse_stepping(qx):
     for _ in range(n):
         b.leapfrog(k * dt, 1)
     assert rel(a.psi, b.psi) < 1e-5
+
+
+@pytest.mark.xfail(strict=False, reason=(
+    "measured at 512^2, pref = 100, dt = 1e-3, k = 4, 1000 training pairs: after 10 coarse steps "
+    "against the dt/4 reference, corrected 1.0e-3, plain coarse 8.8e-4, plain fine 7.8e-5 (gain "
+    "over coarse 0.87); 10 corrected coarse steps take 2.39 ms against 1.50 ms for the 40 fine "
+    "steps they replace (speed-up at equal error 0.16)"))
+def test_trained_corrector_beats_plain_coarse(qx):
+    """The A/B of learned coarse stepping at the config's 512^2: a corrector
+    trained on solver pairs (seeds disjoint from the evaluation) against
+    plain coarse and plain fine stepping, errors against a dt/4 reference
+    inside the predictable horizon (10 coarse steps), with times.  Passes if
+    the network lowers the coarse error and the corrected steps beat plain
+    stepping by 2x at equal error."""
+    from qaxions_amd import learned
+    N, L, dt, k, pref = 512, 10.0, 1e-3, 4, 100.0
+    print("[case] corrector A/B 512^2", flush=True)
+    net, _, relative = learned.train(N, L, dt, k, seeds=range(1000, 1004), steps_per_seed=250, lr=1e-3,
+                                     lr_final=1e-4, pref=pref)
+    res = learned.evaluate(net, N, L, dt, k, seed=0, n_coarse=10, check_every=10, mass_tol=1e-6,
+                           pref=pref, timing_reps=2, warm_steps=200)
+    e = {m: res[m]["rel_l2_vs_reference"] for m in ("fine", "coarse", "learned")}
+    t = {m: res[m]["seconds"] for m in ("fine", "coarse", "learned")}
+    print(f"A/B 512^2: training relative loss {relative[:100].mean():.3f} -> {relative[-100:].mean():.3f}; "
+          f"errors vs dt/4 reference {e}; seconds {t}; gain over coarse {res['error_gain_learned_vs_coarse']:.2f}; "
+          f"speed-up at equal error {res['speedup_learned_vs_plain_at_equal_error']:.2f}")
+    assert res["learned"]["fallback_chunks"] == 0   # the scaled correction keeps the mass
+    assert e["learned"] < e["coarse"]
+    assert res["speedup_learned_vs_plain_at_equal_error"] >= 2.0
+
+
+def test_correction_keeps_the_mass(qx):
+    """The corrected state is scaled back to its mass before the correction:
+    a network far from the identity changes psi, not sum |psi|^2."""
+    from qaxions_amd.learned import init_params
+    N, L = 64, 10.0
+    net = qx.Corrector(N, N, params=init_params(last_scale=0.5))
+    net.output_scale = 0.05
+    lat = qx.Lattice(2, N, L, "fp32", pref=1.0)
+    lat.ic_grf(3)
+    lat.update_potential()
+    p0 = lat.psi
+    net.correct(lat)
+    p1 = lat.psi
+    m0, m1 = np.sum(np.abs(p0) ** 2), np.sum(np.abs(p1) ** 2)
+    assert rel(p1, p0) > 1e-3
+    assert abs(m1 - m0) / m0 < 1e-6
+
+
+def test_energy_gate_falls_back(qx):
+    """With an energy tolerance below what a far-from-identity correction
+    changes, every chunk falls back to fine steps (forced from the host)."""
+    from qaxions_amd.learned import init_params
+    N, L, dt, k = 64, 10.0, 1e-3, 4
+    net = qx.Corrector(N, N, params=init_params(last_scale=0.5))
+    net.output_scale = 0.05
+    net.set_energy_tol(1e-9)
+    a = qx.Lattice(2, N, L, "fp32", pref=1.0)
+    a.ic_grf(5)
+    a.update_potential()
+    b = qx.Lattice(2, N, L, "fp32", pref=1.0)
+    b.copy_from(a)
+    assert net.coarse_steps(a, dt, k, 6, check_every=3, mass_tol=1.0) == 2
+    b.leapfrog(dt, k * 3)
+    b.leapfrog(dt, k * 3)
+    assert np.array_equal(a.psi, b.psi)

@@ -632,6 +632,42 @@ def test_ensemble_members_match_solo_runs(qx, prec, graph_steps):
     assert rel_l2(ens.psi(1), want) < (1e-12 if prec == "fp64" else 5e-6)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_ensemble_config_size_classes(qx, prec):
+    """The batched ensemble's four size classes (128^2, 512^2, 1024^2,
+    2048^2, two members each, own seeds and boxes L in [5, 40], dt scaled to
+    dx^2), graph-captured steps with the tuned tile widths: every member
+    bitwise equal to its solo run, and the 512^2 member against the FP64
+    oracle."""
+    Ns = [128, 512, 1024, 2048, 128, 512, 1024, 2048]
+    Ls = [5.0, 17.5, 40.0, 23.0, 31.0, 8.0, 12.5, 36.0]
+    seeds = [40, 41, 42, 43, 44, 45, 46, 47]
+    dx0 = 10.0 / 256
+    ds = [1e-3 * (L / N / dx0) ** 2 for N, L in zip(Ns, Ls)]
+    steps = 20
+    print(f"[case] ensemble size classes prec={prec}", flush=True)
+    ens = qx.Ensemble(Ns, Ls, precision=prec, pref=1.0, graph_steps=8)
+    ens.ic_grf(seeds)
+    ens.update_potential()
+    ens.tune(ds, reps=2)
+    ens.leapfrog(ds, steps)
+    for i, (N, L) in enumerate(zip(Ns, Ls)):
+        solo = qx.Lattice(2, N, L, precision=prec, pref=1.0)
+        solo.ic_grf(seeds[i])
+        solo.update_potential()
+        solo.leapfrog(ds[i], steps)
+        got, want = ens.psi(i), solo.psi
+        assert np.array_equal(got, want), (i, N, rel_l2(got, want))
+        solo.close()
+    i = 1
+    psi0 = o.ic_grf(2, Ns[i], Ls[i], seeds[i])
+    V0, _, _ = o.update_potential(psi0, Ls[i], 1.0)
+    want, _ = o.leapfrog(psi0, V0, Ls[i], ds[i], steps, 1.0)
+    err = rel_l2(ens.psi(i), want)
+    print(f"512^2 member after {steps} steps against the oracle ({prec}): {err:.2e}")
+    assert err < (1e-12 if prec == "fp64" else 5e-6)
+
+
 def test_pass_timing_and_tune_keep_state(qx):
     """Per-pass timing (inside consecutive steps on scratch copies) and the
     tile-variant tuner leave the state untouched, time every pass, and the
