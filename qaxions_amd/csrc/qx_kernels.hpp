@@ -220,6 +220,14 @@ __device__ __forceinline__ long long vt_index(long long row, int k, long long rp
     const long long mem = row / rpm;
     return mem * (long long)h1 * rpm + (long long)k * rpm + (row - mem * rpm);
 }
+// the same for a block's rows r0 .. r0 + TL - 1 that lie in one member (rpm
+// a multiple of TL, r0 a multiple of TL): vt_index(r0 + l, k) = base + k rpm
+// + l, with the one division per block
+__device__ __forceinline__ long long vt_block_base(long long r0, long long rpm, int h1)
+{
+    const long long mem = r0 / rpm;
+    return mem * (long long)h1 * rpm + (r0 - mem * rpm);
+}
 
 __device__ __forceinline__ int foldi(int i, int N) { return (i <= N / 2) ? i : i - N; }
 
@@ -247,6 +255,16 @@ __device__ __forceinline__ void sincosm1_kick(float x, float *s, float *cm1)
         *cm1 = -2.0f * sh * sh;
     }
 }
+// the polynomial branch alone, for angles known to satisfy |x| <= pi/4
+__device__ __forceinline__ void sincosm1_poly(float x, float *s, float *cm1)
+{
+    const float x2 = x * x;
+    *s = x * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f),
+                           -1.6666667e-1f), 1.0f);
+    *cm1 = x2 * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
+                               4.1666667e-2f), -0.5f);
+}
+
 // FP64: |x| <= pi/4, Taylor polynomials to x^17 / x^18 (truncation below
 // 1e-19 absolute); otherwise one sincos of x/2 gives both, sin x =
 // 2 sin(x/2) cos(x/2) and cos x - 1 = -2 sin^2(x/2)
@@ -280,6 +298,9 @@ __device__ __forceinline__ void sincosm1_kick(double x, double *s, double *cm1)
         *cm1 = -2.0 * sh * sh;
     }
 }
+// FP64: the same function (its polynomial holds to pi/4)
+__device__ __forceinline__ void sincosm1_poly(double x, double *s, double *cm1) { sincosm1_kick(x, s, cm1); }
+
 // psi * e^{ix} = psi + psi (e^{ix} - 1)
 template <typename C, typename R>
 __device__ __forceinline__ C rotate(C a, R s, R cm1)
@@ -783,11 +804,19 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
-        const int k = i / TL, l = i % TL;
-        // streamed also where a block's segment of a k-row is half a line (3D
-        // 1024^3: 3.33 against 3.49 ms plain)
-        if (r0 + l < rows) st<NT>(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
+    // streamed also where a block's segment of a k-row is half a line (3D
+    // 1024^3: 3.33 against 3.49 ms plain)
+    if (rpm % TL == 0) {
+        C *const vb = vt + vt_block_base(r0, rpm, H1);
+        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+            const int k = i / TL, l = i % TL;
+            if (r0 + l < rows) st<NT>(vb + (long long)k * rpm + l, stg[k * G::STP + l]);
+        }
+    } else {
+        for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
+            const int k = i / TL, l = i % TL;
+            if (r0 + l < rows) st<NT>(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
+        }
     }
 }
 
@@ -859,11 +888,21 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         C *stg = lds;
         constexpr int NS = (H1 * TL + G::ROW_THREADS_C - 1) / G::ROW_THREADS_C;
         C tmp[NS];
+        if (rpm % TL == 0) {
+            const C *const vb = vt + vt_block_base(r0, rpm, H1);
 #pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int i = threadIdx.x + j * G::ROW_THREADS_C;
-            const int k = i / TL, l = i % TL;
-            tmp[j] = (i < H1 * TL && r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
+            for (int j = 0; j < NS; ++j) {
+                const int i = threadIdx.x + j * G::ROW_THREADS_C;
+                const int k = i / TL, l = i % TL;
+                tmp[j] = (i < H1 * TL && r0 + l < rows) ? vb[(long long)k * rpm + l] : C{0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const int i = threadIdx.x + j * G::ROW_THREADS_C;
+                const int k = i / TL, l = i % TL;
+                tmp[j] = (i < H1 * TL && r0 + l < rows) ? vt[vt_index(r0 + l, k, rpm, H1)] : C{0, 0};
+            }
         }
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
@@ -922,13 +961,29 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         if (flags & C2R_KICK) {
             const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
             const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
+            // one test for all of this thread's angles: the polynomials cover
+            // |x| <= pi/4, which the method's step sizes keep to
+            R big = 0;
 #pragma unroll
-            for (int q = 0; q < P::E; ++q) {
-                R sa, ca, sb, cb;
-                sincosm1_kick(-dta * va[q], &sa, &ca);
-                sincosm1_kick(-dtb * vb[q], &sb, &cb);
-                a[q] = rotate(a[q], sa, ca);
-                b[q] = rotate(b[q], sb, cb);
+            for (int q = 0; q < P::E; ++q) big = fmax(big, fmax(fabs(dta * va[q]), fabs(dtb * vb[q])));
+            if (big <= (R)0.78539816339744831) {
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    R sa, ca, sb, cb;
+                    sincosm1_poly(-dta * va[q], &sa, &ca);
+                    sincosm1_poly(-dtb * vb[q], &sb, &cb);
+                    a[q] = rotate(a[q], sa, ca);
+                    b[q] = rotate(b[q], sb, cb);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    R sa, ca, sb, cb;
+                    sincosm1_kick(-dta * va[q], &sa, &ca);
+                    sincosm1_kick(-dtb * vb[q], &sb, &cb);
+                    a[q] = rotate(a[q], sa, ca);
+                    b[q] = rotate(b[q], sb, cb);
+                }
             }
         }
         if (flags & C2R_DRIFT) {
