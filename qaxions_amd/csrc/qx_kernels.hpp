@@ -443,10 +443,19 @@ enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 // Waves per SIMD that a strided FP32 drift block's LDS allows: its two-word
 // twiddles raise the register count, and the hint keeps it within what that
 // occupancy leaves (1 = no hint for every other pass)
+// Two-word twiddles (cmul2) in the FP32 drift transforms of lines up to 512
+// points.  Longer lines keep one word: their three twiddled stages and
+// their LDS and registers make the second word cost 11% of the 1024^3 step
+// (23.6 -> 26.3 ms), where psi's FP32 error over the 50-step workload stays
+// within 1e-5 with one word.
+template <int N, typename R> struct TwoWord {
+    static constexpr bool value = std::is_same<R, float>::value && N <= 512;
+};
+
 template <int N, typename R, int OP, int TC, int CPT> struct StridedWaves {
-    static constexpr bool ON = OP == OP_DRIFT_AXIS && std::is_same<R, float>::value;
-    // exchange area and two-word twiddles; the FP32 drift table where it fits
-    static constexpr long long LDS0 = 8ll * N * (TC / CPT) + 16ll * N;
+    static constexpr bool ON = OP == OP_DRIFT_AXIS && TwoWord<N, R>::value;
+    // exchange area and twiddles; the FP32 drift table where it fits
+    static constexpr long long LDS0 = 8ll * N * (TC / CPT) + (ON ? 16ll : 8ll) * N;
     static constexpr bool FTAB_LDS = LDS0 + 8ll * N <= 81920;
     static constexpr long long LDS = LDS0 + (FTAB_LDS ? 8ll * N : 0);
     static constexpr int W0 = (int)(163840 / LDS) * ((TC / CPT) * Plan<N>::TPL / 64) / 4;
@@ -464,19 +473,20 @@ strided_fft(typename cplx<R>::T *__restrict__ data, long long inner, long long n
     using P = Plan<N>;
     constexpr int TW = TC / CPT;
     // FP32 drift: the transforms act on psi every step and take two-word
-    // twiddles (TW_TWO), interleaved in one table
+    // twiddles (TW_TWO, interleaved in one table) where TwoWord holds
     constexpr bool FAX = OP == OP_DRIFT_AXIS && std::is_same<R, float>::value;
-    constexpr int TWM = FAX ? TW_TWO : TW_PLAIN;
+    constexpr bool XTW = FAX && TwoWord<N, R>::value;
+    constexpr int TWM = XTW ? TW_TWO : TW_PLAIN;
     __shared__ C lds[N * TW];
-    __shared__ __align__(16) C tw[FAX ? 2 * N : N];
+    __shared__ __align__(16) C tw[XTW ? 2 * N : N];
     // the drift's phase table is staged in LDS when small: an FP32
     // single-axis drift stages its FP32 table where two blocks still fit a
     // CU (otherwise each thread reads its entries from global memory, where
     // they stay cached; loading them ahead into registers would cost the
     // second block)
-    constexpr bool PT_LDS = OP == OP_DRIFT_AXIS && (FAX ? StridedWaves<N, R, OP, TC, CPT>::FTAB_LDS : N <= 1024);
+    constexpr bool PT_LDS = OP == OP_DRIFT_AXIS && (XTW ? StridedWaves<N, R, OP, TC, CPT>::FTAB_LDS : N <= 1024);
     __shared__ double2 ptab_lds[PT_LDS ? (FAX ? N / 2 : N) : 1];
-    if constexpr (FAX) load_tw2((float2 *)tw, (const float2 *)twg, N);
+    if constexpr (XTW) load_tw2((float2 *)tw, (const float2 *)twg, N);
     else load_tw<N>(tw, twg);
     const int c = threadIdx.x % TW, t = threadIdx.x / TW;
     const long long b = xcd_block();
@@ -849,10 +859,10 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     using G = Geom<N, R>;
     constexpr int TL = G::TLC, H1 = N / 2 + 1;
     __shared__ C lds[G::ROW_LDS];
-    // FP32: twiddles with their low words, interleaved; the drift's
-    // transforms act on psi every step and take both (TW_TWO), the
+    // FP32 (TwoWord): twiddles with their low words, interleaved; the
+    // drift's transforms act on psi every step and take both (TW_TWO), the
     // potential's transform the rounded twiddles alone (TW_HI)
-    constexpr bool XT = std::is_same<R, float>::value;
+    constexpr bool XT = TwoWord<N, R>::value;
     constexpr int TWH = XT ? TW_HI : TW_PLAIN, TWD = XT ? TW_TWO : TW_PLAIN;
     __shared__ __align__(16) C tw[XT ? 2 * N : N];
     __shared__ double wmax[16];
