@@ -440,9 +440,6 @@ enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 // 16-column tile's 128-byte row segments with 512 threads per block).
 // NT: streaming accesses where a block row is a whole 128 B line (narrower
 // rows share lines between blocks, which then merge in L2)
-// Waves per SIMD that a strided FP32 drift block's LDS allows: its two-word
-// twiddles raise the register count, and the hint keeps it within what that
-// occupancy leaves (1 = no hint for every other pass)
 // Two-word twiddles (cmul2) in the FP32 drift transforms of lines up to 512
 // points.  Longer lines keep one word: their three twiddled stages and
 // their LDS and registers make the second word cost 11% of the 1024^3 step
@@ -452,6 +449,9 @@ template <int N, typename R> struct TwoWord {
     static constexpr bool value = std::is_same<R, float>::value && N <= 512;
 };
 
+// Waves per SIMD that a strided FP32 drift block's LDS allows: its two-word
+// twiddles raise the register count, and the hint keeps it within what that
+// occupancy leaves (1 = no hint for every other pass)
 template <int N, typename R, int OP, int TC, int CPT> struct StridedWaves {
     static constexpr bool ON = OP == OP_DRIFT_AXIS && TwoWord<N, R>::value;
     // exchange area and twiddles; the FP32 drift table where it fits
