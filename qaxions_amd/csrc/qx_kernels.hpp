@@ -265,6 +265,15 @@ __device__ __forceinline__ void sincosm1_poly(float x, float *s, float *cm1)
                                4.1666667e-2f), -0.5f);
 }
 
+// |x| <= 2^-6: sin x = x - x^3/6 and cos x - 1 = -x^2/2 + x^4/24, with
+// truncation below x^4/120 <= 5e-10 and x^4/360 <= 2e-10 relative
+__device__ __forceinline__ void sincosm1_short(float x, float *s, float *cm1)
+{
+    const float x2 = x * x;
+    *s = fmaf(x * x2, -1.6666667e-1f, x);
+    *cm1 = x2 * fmaf(x2, 4.1666667e-2f, -0.5f);
+}
+
 // FP64: |x| <= pi/4, Taylor polynomials to x^17 / x^18 (truncation below
 // 1e-19 absolute); otherwise one sincos of x/2 gives both, sin x =
 // 2 sin(x/2) cos(x/2) and cos x - 1 = -2 sin^2(x/2)
@@ -300,6 +309,8 @@ __device__ __forceinline__ void sincosm1_kick(double x, double *s, double *cm1)
 }
 // FP64: the same function (its polynomial holds to pi/4)
 __device__ __forceinline__ void sincosm1_poly(double x, double *s, double *cm1) { sincosm1_kick(x, s, cm1); }
+// FP64 keeps the long polynomials at every angle (the short branch is FP32's)
+__device__ __forceinline__ void sincosm1_short(double x, double *s, double *cm1) { sincosm1_kick(x, s, cm1); }
 
 // psi * e^{ix} = psi + psi (e^{ix} - 1)
 template <typename C, typename R>
@@ -972,11 +983,21 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             const R dta = kick_member ? (R)kick_member[oka ? ra / rpm : 0] : kick_dt;
             const R dtb = kick_member ? (R)kick_member[okb ? rb / rpm : 0] : kick_dt;
             // one test for all of this thread's angles: the polynomials cover
-            // |x| <= pi/4, which the method's step sizes keep to
+            // |x| <= pi/4, which the method's step sizes keep to, and the
+            // short ones |x| <= 2^-6 (FP32), the usual range
             R big = 0;
 #pragma unroll
             for (int q = 0; q < P::E; ++q) big = fmax(big, fmax(fabs(dta * va[q]), fabs(dtb * vb[q])));
-            if (big <= (R)0.78539816339744831) {
+            if (std::is_same<R, float>::value && big <= (R)0.015625) {
+#pragma unroll
+                for (int q = 0; q < P::E; ++q) {
+                    R sa, ca, sb, cb;
+                    sincosm1_short(-dta * va[q], &sa, &ca);
+                    sincosm1_short(-dtb * vb[q], &sb, &cb);
+                    a[q] = rotate(a[q], sa, ca);
+                    b[q] = rotate(b[q], sb, cb);
+                }
+            } else if (big <= (R)0.78539816339744831) {
 #pragma unroll
                 for (int q = 0; q < P::E; ++q) {
                     R sa, ca, sb, cb;
@@ -999,7 +1020,11 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
         if (flags & C2R_DRIFT) {
             const long long ma_ = (oka ? ra / rpm : 0) * ptab_stride, mb_ = (okb ? rb / rpm : 0) * ptab_stride;
             C *mine_b = lds + (G::NPC + p) * P::PADDED;
-            __syncthreads();
+            // the twiddles loaded at the start are visible after the
+            // staging's barrier; without staging, after this one.  A pair's
+            // exchange lines are its own, and the staging area's last reads
+            // precede the potential transform's barrier
+            if (!(flags & C2R_SRC_VT)) __syncthreads();
             if constexpr (G::TWO_LINES) {
                 fft_line2<N, -1, LdsContig, TWD>(a, b, t, mine, mine_b, 1, tw);
             } else {
