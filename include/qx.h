@@ -227,6 +227,10 @@ qx_status qx_ens_leapfrog(qx_ens *ens, const double *ds, int nsteps, int first_h
  * class on saved copies of its state (the state is restored).  variants
  * (nullable): one int per class, classes in increasing N */
 qx_status qx_ens_tune(qx_ens *ens, const double *ds, int reps, int *variants);
+/* Reinstall a tuning result (one variant per class, as qx_ens_tune returns
+ * them: 0 full tile, 1 half tile), e.g. one kept from an earlier run.  The
+ * choice changes speed, never results. */
+qx_status qx_ens_set_tile_variants(qx_ens *ens, const int *variants);
 int qx_ens_classes(const qx_ens *ens);
 int qx_ens_class_size(const qx_ens *ens, int cls);
 /* mass[member] = (L/N)^2 sum |psi|^2 */

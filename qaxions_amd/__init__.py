@@ -100,6 +100,7 @@ SIGNATURES = {
     "qx_ens_update_potential": (C.c_int, [C.c_void_p]),
     "qx_ens_leapfrog": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int]),
     "qx_ens_tune": (C.c_int, [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "qx_ens_set_tile_variants": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "qx_ens_classes": (C.c_int, [C.c_void_p]),
     "qx_ens_class_size": (C.c_int, [C.c_void_p, C.c_int]),
     "qx_ens_mass": (C.c_int, [C.c_void_p, _dp]),
@@ -476,6 +477,7 @@ class Ensemble:
                  rho_bar_mode=0, graph_steps=8):
         self.Ns = [int(n) for n in Ns]
         self.Ls = [float(x) for x in Ls]
+        self.precision = precision
         n = len(self.Ns)
         h = C.c_void_p()
         _check(lib().qx_ens_create(n, (C.c_int * n)(*self.Ns), (C.c_double * n)(*self.Ls),
@@ -527,14 +529,45 @@ class Ensemble:
     def update_potential(self):
         _check(lib().qx_ens_update_potential(self._h))
 
-    def tune(self, ds, reps=3):
-        """Autotune each size class's drift tile by timing its steps (the
-        state is kept); returns {N: variant} (0 full tile, 1 half tile)"""
-        d = np.ascontiguousarray(np.broadcast_to(np.asarray(ds, np.float64), (len(self.Ns),)))
+    def _classes(self):
         n = int(lib().qx_ens_classes(self._h))
-        v = (C.c_int * max(n, 1))()
+        return [int(lib().qx_ens_class_size(self._h, i)) for i in range(n)]
+
+    def tune(self, ds, reps=3, cache=None):
+        """Autotune each size class's drift tile by timing its steps (the
+        state is kept); returns {N: variant} (0 full tile, 1 half tile).
+        With a cache path (JSON), a result stored for the same class make-up
+        (sizes, member counts, precision) is reinstalled instead of timed,
+        else the new result is stored."""
+        import json
+        sizes = self._classes()
+        key = "ens-" + self.precision + "-" + ",".join(f"{n}x{list(self.Ns).count(n)}" for n in sizes)
+        db = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                db = json.load(f)
+            if key in db:
+                got = {int(k): int(v) for k, v in db[key].items()}
+                self.set_tile_variants(got)
+                return got
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(ds, np.float64), (len(self.Ns),)))
+        v = (C.c_int * max(len(sizes), 1))()
         _check(lib().qx_ens_tune(self._h, _p(d), int(reps), v))
-        return {int(lib().qx_ens_class_size(self._h, i)): int(v[i]) for i in range(n)}
+        out = {n: int(v[i]) for i, n in enumerate(sizes)}
+        if cache:
+            db[key] = {str(k): x for k, x in out.items()}
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(db, f, indent=1)
+        return out
+
+    def set_tile_variants(self, variants):
+        """Reinstall {N: variant} (or one variant per class in increasing N)."""
+        sizes = self._classes()
+        vals = [variants[n] for n in sizes] if isinstance(variants, dict) else list(variants)
+        if len(vals) != len(sizes):
+            raise ValueError(f"{len(sizes)} size classes, {len(vals)} variants")
+        _check(lib().qx_ens_set_tile_variants(self._h, (C.c_int * max(len(vals), 1))(*vals)))
 
     def leapfrog(self, ds, nsteps, first_half=True, last_half=True):
         n = len(self.Ns)

@@ -802,7 +802,8 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     fft_line<N, -1, LdsContig>(z, t, mine, 1, tw);
 #pragma unroll
     for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = z[q];
-    __syncthreads();
+    // the pair's own line: a wave-local wait where one wavefront holds it
+    line_sync<N, LdsContig, P::E>();
     C oa[P::E / 2 + 1], ob[P::E / 2 + 1];
 #pragma unroll
     for (int q = 0; q <= P::E / 2; ++q) {

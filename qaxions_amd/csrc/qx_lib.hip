@@ -1932,6 +1932,17 @@ qx_status qx_ens_tune(qx_ens *e, const double *ds, int reps, int *variants)
     return st;
 }
 
+qx_status qx_ens_set_tile_variants(qx_ens *e, const int *variants)
+{
+    if (!e || !variants) return fail(QX_EINVAL, "null argument");
+    for (size_t ci = 0; ci < e->classes.size(); ++ci)
+        if (variants[ci] != 0 && variants[ci] != 1) return fail(QX_EINVAL, "tile variant must be 0 or 1");
+    ens_free_graph(e);   // captured launches would keep the old tiles
+    for (size_t ci = 0; ci < e->classes.size(); ++ci)
+        for (auto &row : e->classes[ci].eng->strided_variant) row[0] = row[1] = variants[ci];
+    return QX_OK;
+}
+
 int qx_ens_classes(const qx_ens *e) { return e ? (int)e->classes.size() : 0; }
 int qx_ens_class_size(const qx_ens *e, int ci)
 {

@@ -1,6 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the FP64 oracle on the same
 seeded inputs.  Tolerances: FP64 contexts 1e-12 relative, FP32 contexts as
 stated per test (the FP32 bar: <= 1e-5 relative L2 after 100 steps)."""
+import json
 import math
 
 import numpy as np
@@ -666,6 +667,44 @@ def test_ensemble_config_size_classes(qx, prec):
     err = rel_l2(ens.psi(i), want)
     print(f"512^2 member after {steps} steps against the oracle ({prec}): {err:.2e}")
     assert err < (1e-12 if prec == "fp64" else 5e-6)
+
+
+def test_ensemble_tune_cache(qx, tmp_path):
+    """The ensemble tuner's cache: the first tune stores {N: variant} under
+    the class make-up, a second ensemble of that make-up reinstalls the
+    stored choice without timing (a forged entry comes back as stored), and
+    stepping after either equals an untuned ensemble bit for bit."""
+    Ns, Ls = [64, 128, 64, 128, 64], [10.0, 20.0, 12.0, 25.0, 8.0]
+    ds = [1e-3] * len(Ns)
+    cache = str(tmp_path / "ens_tiles.json")
+    print("[case] ensemble tune cache", flush=True)
+
+    def run(tune):
+        e = qx.Ensemble(Ns, Ls, precision="fp32", pref=1.0, graph_steps=4)
+        e.ic_grf(list(range(7, 7 + len(Ns))))
+        e.update_potential()
+        got = tune(e) if tune else None
+        e.leapfrog(ds, 12)
+        return got, [e.psi(i) for i in range(len(Ns))]
+
+    _, plain = run(None)
+    first, a = run(lambda e: e.tune(ds, reps=2, cache=cache))
+    assert set(first) == {64, 128} and set(first.values()) <= {0, 1}
+    with open(cache) as f:
+        db = json.load(f)
+    key = "ens-fp32-64x3,128x2"
+    assert db[key] == {str(k): v for k, v in first.items()}
+    forged = {64: 1 - first[64], 128: 1 - first[128]}
+    db[key] = {str(k): v for k, v in forged.items()}
+    with open(cache, "w") as f:
+        json.dump(db, f)
+    second, b = run(lambda e: e.tune(ds, reps=2, cache=cache))
+    assert second == forged
+    for x, y, z in zip(plain, a, b):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    with pytest.raises(Exception):
+        e = qx.Ensemble(Ns, Ls, precision="fp32")
+        e.set_tile_variants({64: 2, 128: 0})
 
 
 def test_pass_timing_and_tune_keep_state(qx):
