@@ -39,7 +39,7 @@ struct Engine {
     // [k1][i0]); `units` of work (column tiles or line blocks) for the grid
     virtual long long bin_units(int dim) const = 0;
     virtual void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) = 0;
-    void *tw = nullptr;  // device twiddles e^{-2 pi i m/N}, m < N (FP32: then their low words, see cmul2)
+    void *tw = nullptr;  // device twiddles in stage-major order, N entries (FP32: then their low words, see cmul2)
     // streaming loads and stores in the passes (lattices whose arrays are far
     // larger than the memory-side cache; instantiated for N >= 512 only)
     bool stream_io = false;

@@ -42,11 +42,11 @@ __device__ __forceinline__ float2 cmul2(float2 a, float2 wh, float2 wl)
 }
 __device__ __forceinline__ double2 cmul2(double2 a, double2 wh, double2) { return cmul(a, wh); }
 
-// Twiddle tables in LDS.  TW_PLAIN: tw[m], one word per twiddle.  FP32
-// passes that transform psi keep them interleaved, tw[2m] the rounded
-// twiddle and tw[2m + 1] its low word (one 16-byte read per twiddle):
-// TW_HI reads only the rounded twiddles of such a table, TW_TWO both words
-// (two-word products, cmul2).
+// Twiddle tables in LDS, entry m in stage-major order (twiddle_offset).
+// TW_PLAIN: tw[m], one word per twiddle.  FP32 passes that transform psi
+// keep them interleaved, tw[2m] the rounded twiddle and tw[2m + 1] its low
+// word (one 16-byte read per twiddle): TW_HI reads only the rounded
+// twiddles of such a table, TW_TWO both words (two-word products, cmul2).
 enum { TW_PLAIN = 0, TW_HI = 1, TW_TWO = 2 };
 template <int TWM, int DIR, typename C>
 __device__ __forceinline__ C twiddle_mul(C u, const C *tw, int m)
@@ -139,17 +139,38 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
+// the stage structure of an N-point transform, shared by host and device
+__host__ __device__ constexpr int ipow8(int k) { return k == 0 ? 1 : 8 * ipow8(k - 1); }
+__host__ __device__ constexpr int plan_stages(int N) { return (ilog2(N) + 2) / 3; }
+__host__ __device__ constexpr int plan_r0(int N) { return (ilog2(N) % 3 == 0) ? 8 : ((ilog2(N) % 3 == 1) ? 2 : 4); }
+__host__ __device__ constexpr int plan_radix(int N, int s) { return N < 8 ? N : (s == 0 ? plan_r0(N) : 8); }
+__host__ __device__ constexpr int plan_ns(int N, int s) { return s == 0 ? 1 : (N < 8 ? 1 : plan_r0(N) * ipow8(s - 1)); }
+
+// Stage-major twiddle table: stage s (sub-transform length Ns > 1, radix R,
+// table stride TWS = N/(Ns R)) keeps its twiddles w^{r k TWS}, r = 1..R-1,
+// k = 0..Ns-1, at twiddle_offset(N, s) + (r - 1) Ns + k, with w = e^{-2 pi
+// i/N}.  The lanes of a wavefront then read consecutive entries (k follows
+// the thread index), free of LDS bank conflicts, where the natural order
+// w^m put the reads of a stage r k TWS entries apart.  The table holds
+// N - R0 entries.
+__host__ __device__ constexpr int twiddle_offset(int N, int s)
+{
+    int off = 0;
+    for (int i = 0; i < s; ++i)
+        if (plan_ns(N, i) > 1) off += (plan_radix(N, i) - 1) * plan_ns(N, i);
+    return off;
+}
+
 // EV values per thread: 8 by default (16 for the long contiguous lines of
 // the row kernels, so that a line of 1024 points is one wavefront)
 template <int N, int EV = (N < 8 ? N : 8)> struct Plan {
     static constexpr int E = EV;                 // values per thread
     static constexpr int TPL = N / E;            // threads per line
     static constexpr int LG = ilog2(N);
-    static constexpr int NST = (LG + 2) / 3;     // number of stages
-    static constexpr int R0 = (LG % 3 == 0) ? 8 : ((LG % 3 == 1) ? 2 : 4);
-    static constexpr int radix(int s) { return N < 8 ? N : (s == 0 ? R0 : 8); }
-    static constexpr int ns(int s) { return s == 0 ? 1 : (N < 8 ? 1 : R0 * ipow8(s - 1)); }
-    static constexpr int ipow8(int k) { return k == 0 ? 1 : 8 * ipow8(k - 1); }
+    static constexpr int NST = plan_stages(N);   // number of stages
+    static constexpr int R0 = plan_r0(N);
+    static constexpr int radix(int s) { return plan_radix(N, s); }
+    static constexpr int ns(int s) { return plan_ns(N, s); }
     // padded LDS slots for one contiguous line (one pad slot per 8 values)
     static constexpr int PADDED = N + N / 8 + 1;
 };
@@ -186,7 +207,7 @@ __device__ __forceinline__ void fft_stage(C (&v)[EV], int t, C *line, int tc, co
     constexpr int E = P::E, TPL = P::TPL;
     constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
     constexpr bool last = (S == P::NST - 1);
-    constexpr int TWS = N / (Ns * R);   // twiddle table stride
+    constexpr int TWO = twiddle_offset(N, S);   // this stage's table
     C out[E];
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
@@ -197,7 +218,7 @@ __device__ __forceinline__ void fft_stage(C (&v)[EV], int t, C *line, int tc, co
         for (int r = 0; r < R; ++r) u[r] = v[m + r * NB];
         if (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) u[r] = twiddle_mul<TWM, DIR>(u[r], tw, r * k * TWS);
+            for (int r = 1; r < R; ++r) u[r] = twiddle_mul<TWM, DIR>(u[r], tw, TWO + (r - 1) * Ns + k);
         }
         Dft<R, DIR, C>::run(u);
         if (last) {
@@ -233,7 +254,7 @@ __device__ __forceinline__ void stage_loop(C (&v)[EV], int t, C *line, int tc, c
 // must call it (column lines and lines of more than 64 threads synchronise
 // with block barriers).  `line` is this line's LDS
 // area, `tc` the column interleave for LdsCols, `tw` the LDS twiddle table
-// tw[m] = e^{-2 pi i m/N} (TW_PLAIN), or that table interleaved with the
+// in stage-major order (TW_PLAIN), or that table interleaved with the
 // twiddles' low words (TW_HI, TW_TWO).
 template <int N, int DIR, typename L, int TWM = TW_PLAIN, typename C, int EV>
 __device__ __forceinline__ void fft_line(C (&v)[EV], int t, C *line, int tc, const C *tw)
@@ -253,7 +274,7 @@ __device__ __forceinline__ void fft_stage2(C (&va)[EV], C (&vb)[EV], int t, C *l
     constexpr int E = P::E, TPL = P::TPL;
     constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
     constexpr bool last = (S == P::NST - 1);
-    constexpr int TWS = N / (Ns * R);
+    constexpr int TWO = twiddle_offset(N, S);
     C oa[E], ob[E];
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
@@ -268,8 +289,8 @@ __device__ __forceinline__ void fft_stage2(C (&va)[EV], C (&vb)[EV], int t, C *l
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                ua[r] = twiddle_mul<TWM, DIR>(ua[r], tw, r * k * TWS);
-                ub[r] = twiddle_mul<TWM, DIR>(ub[r], tw, r * k * TWS);
+                ua[r] = twiddle_mul<TWM, DIR>(ua[r], tw, TWO + (r - 1) * Ns + k);
+                ub[r] = twiddle_mul<TWM, DIR>(ub[r], tw, TWO + (r - 1) * Ns + k);
             }
         }
         Dft<R, DIR, C>::run(ua);

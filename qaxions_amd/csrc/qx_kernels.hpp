@@ -77,9 +77,9 @@ template <int N, typename R> struct Geom {
     static constexpr int ROW_LDS1 = XCH1 > STG ? XCH1 : STG;    // r2c_rows
 
     // c2r_rows.  Where a line is one wavefront (TPL = 64) the kernel is
-    // latency-limited and its registers allow 5 waves per SIMD: it takes
-    // blocks of 4 waves with one exchange line per pair (22 KB of LDS), so
-    // that 5 blocks fit a CU.  Lines of two wavefronts (N = 1024) also take
+    // latency-limited: it takes blocks of 4 waves with one exchange line per
+    // pair (27 KB of LDS with the two-word twiddles), so that 6 blocks fit a
+    // CU (C2R_WAVES).  Lines of two wavefronts (N = 1024) also take
     // 4-wave blocks, so that each exchange's block barrier waits for 4 waves
     // instead of 8.  Otherwise it keeps the r2c_rows blocks and transforms a
     // pair's two psi rows in lockstep, each with its own exchange line, where
@@ -95,10 +95,10 @@ template <int N, typename R> struct Geom {
     static constexpr int STPC = TLC >= 8 ? TLC + 1 : TLC;
     static constexpr int STGC = (N / 2 + 1) * STPC;
     static constexpr int ROW_LDS = XCH > STGC ? XCH : STGC;
-    // occupancy kept for the latency-limited single-wave-line c2r_rows
-    // (FP32): 5 waves per SIMD, as its registers allowed before the drift's
-    // two-word twiddles
-    static constexpr int C2R_WAVES = (WAVE_LINE && sizeof(C) == 8) ? 5 : 1;
+    // occupancy of the latency-limited single-wave-line c2r_rows (FP32): 6
+    // waves per SIMD, 80 VGPRs with a few spilled (at 512^3 20 us faster than
+    // 5 waves with none)
+    static constexpr int C2R_WAVES = (WAVE_LINE && sizeof(C) == 8) ? 6 : 1;
 };
 
 // pointwise factor in fused passes

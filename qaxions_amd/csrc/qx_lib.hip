@@ -59,6 +59,21 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 std::unique_ptr<Engine> make_engine(int N, bool fp64) { return fp64 ? make_engine_f64(N) : make_engine_f32(N); }
 
+// natural index m (w^m) of each entry of the stage-major table
+// (twiddle_offset); entries past the last stage stay unused
+std::vector<int> twiddle_order(int N)
+{
+    std::vector<int> order;
+    for (int s = 0; s < plan_stages(N); ++s) {
+        const int R = plan_radix(N, s), Ns = plan_ns(N, s);
+        if (Ns == 1) continue;
+        const int tws = N / (Ns * R);
+        for (int r = 1; r < R; ++r)
+            for (int k = 0; k < Ns; ++k) order.push_back(r * k * tws);
+    }
+    return order;
+}
+
 // Twiddles e^{-2 pi i m/N}, computed in FP64.  In FP32 each is rounded to
 // the pair of neighbouring floats (cos and sin each rounded down or up) whose
 // squared modulus is closest to 1: nearest rounding of the two parts leaves
@@ -66,18 +81,25 @@ std::unique_ptr<Engine> make_engine(int N, bool fp64) { return fp64 ? make_engin
 // every twiddled multiply would add to the norm (mass) of psi.  FP32 tables
 // hold N more entries after those: the remainders w - wh of the rounded
 // twiddles wh, rounded, which the drift's transforms add in (two-word
-// products, cmul2).
+// products, cmul2).  Both are stored in stage-major order (twiddle_order).
 qx_status upload_twiddles(void *dst, int N, bool fp64)
 {
-    std::vector<double> tw(2 * (size_t)N);
+    std::vector<double> nat(2 * (size_t)N);
     for (int m = 0; m < N; ++m) {
         const double a = TWO_PI * (double)m / (double)N;
         double cr = std::cos(a), ci = -std::sin(a);
         if (4 * m == N) { cr = 0.0; ci = -1.0; }
         if (2 * m == N) { cr = -1.0; ci = 0.0; }
         if (4 * m == 3 * N) { cr = 0.0; ci = 1.0; }
-        tw[2 * m] = cr;
-        tw[2 * m + 1] = ci;
+        nat[2 * m] = cr;
+        nat[2 * m + 1] = ci;
+    }
+    const std::vector<int> order = twiddle_order(N);
+    if (order.size() > (size_t)N) return fail(QX_EDEVICE, "twiddle table longer than N");
+    std::vector<double> tw(2 * (size_t)N, 0.0);
+    for (size_t i = 0; i < order.size(); ++i) {
+        tw[2 * i] = nat[2 * (size_t)order[i]];
+        tw[2 * i + 1] = nat[2 * (size_t)order[i] + 1];
     }
     hipError_t e;
     if (fp64) {
