@@ -1004,7 +1004,8 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
     if (!Pk || !count || !kmodes) return fail(QX_EINVAL, "null output");
     const int nsh = c->N / 2 + 1;
     TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
-    TRY(ensure(c, (void **)&c->spec_part, sizeof(double) * SPEC_BLOCKS * nsh));
+    const unsigned grid = (unsigned)std::min<long long>(SPEC_BLOCKS, c->eng->bin_units(c->dim));
+    TRY(ensure(c, (void **)&c->spec_part, sizeof(double) * grid * nsh));
     TRY(ensure(c, (void **)&c->spec_P, sizeof(double) * nsh));
     TRY(ensure(c, (void **)&c->spec_cnt, sizeof(unsigned long long) * nsh));
     // the field's half spectrum: r2c along the last axis (of V, or of |psi|^2
@@ -1016,7 +1017,6 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
         c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     MulArgs none{};
     if (c->dim == 3) c->eng->contig(c->aux_vt, c->H1 * c->N, -1, none, c->stream);
-    const unsigned grid = (unsigned)std::min<long long>(SPEC_BLOCKS, c->eng->bin_units(c->dim));
     const BinArgs ba{nbins, spb, c->spec_part};
     c->eng->spectrum_bin(c->dim, c->aux_vt, ba, grid, c->stream);
     bin_reduce<<<nbins, 256, 0, c->stream>>>(c->spec_part, (int)grid, nbins, c->spec_P);
