@@ -79,26 +79,29 @@ template <int N, typename R> struct Geom {
     // c2r_rows.  Where a line is one wavefront (TPL = 64) the kernel is
     // latency-limited: it takes blocks of 4 waves with one exchange line per
     // pair (27 KB of LDS with the two-word twiddles), so that 6 blocks fit a
-    // CU (C2R_WAVES).  Lines of two wavefronts (N = 1024) also take
-    // 4-wave blocks, so that each exchange's block barrier waits for 4 waves
-    // instead of 8.  Otherwise it keeps the r2c_rows blocks and transforms a
-    // pair's two psi rows in lockstep, each with its own exchange line, where
-    // two lines per pair fit in LDS beside the twiddles (else one after the
-    // other through one line).
+    // CU (C2R_WAVES).  Lines of two wavefronts (N = 1024) also take 4-wave
+    // blocks, so that each exchange's block barrier waits for 4 waves instead
+    // of 8, with one exchange line per pair (27 KB), so that 5 blocks fit
+    // (two lines in lockstep, 45 KB, left 3).  Otherwise it keeps the
+    // r2c_rows blocks and transforms a pair's two psi rows in lockstep, each
+    // with its own exchange line, where two lines per pair fit in LDS beside
+    // the twiddles (else one after the other through one line).
     static constexpr bool WAVE_LINE = PR::TPL == 64;
-    static constexpr int NPC = (WAVE_LINE && NP > 4) ? 4 : ((PR::TPL == 128 && NP > 2) ? 2 : NP);
+    static constexpr bool PAIR_LINE = PR::TPL == 128;
+    static constexpr int NPC = (WAVE_LINE && NP > 4) ? 4 : ((PAIR_LINE && NP > 2) ? 2 : NP);
     static constexpr int TLC = 2 * NPC;               // rows per block
     static constexpr int ROW_THREADS_C = NPC * PR::TPL;
-    static constexpr bool TWO_LINES =
-        !WAVE_LINE && (long long)(2 * NPC * PR::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
+    static constexpr bool TWO_LINES = !WAVE_LINE && !PAIR_LINE &&
+        (long long)(2 * NPC * PR::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
     static constexpr int XCH = (TWO_LINES ? 2 : 1) * NPC * PR::PADDED;  // exchange slots
     static constexpr int STPC = TLC >= 8 ? TLC + 1 : TLC;
     static constexpr int STGC = (N / 2 + 1) * STPC;
     static constexpr int ROW_LDS = XCH > STGC ? XCH : STGC;
-    // occupancy of the latency-limited single-wave-line c2r_rows (FP32): 6
+    // occupancy of the latency-limited FP32 c2r_rows: one-wavefront lines 6
     // waves per SIMD, 80 VGPRs with a few spilled (at 512^3 20 us faster than
-    // 5 waves with none)
-    static constexpr int C2R_WAVES = (WAVE_LINE && sizeof(C) == 8) ? 6 : 1;
+    // 5 waves with none); two-wavefront lines 5 (90 VGPRs; 6 spilled 25 and
+    // took 6.00 against 5.54 ms at 1024^3)
+    static constexpr int C2R_WAVES = sizeof(C) != 8 ? 1 : (WAVE_LINE ? 6 : (PAIR_LINE ? 5 : 1));
 };
 
 // pointwise factor in fused passes
