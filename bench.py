@@ -185,6 +185,10 @@ def main():
                          "lattices; learned: corrector training and evaluation at 512^2; "
                          "copy: device-to-device state copies of known size (the reference "
                          "dispatch for scaling HBM traffic counters)")
+    ap.add_argument("--graph-steps", type=int, default=16,
+                    help="ensemble: steps per captured graph (0: every pass launched directly)")
+    ap.add_argument("--compare-solo", action="store_true",
+                    help="ensemble: also time each member stepped alone in its own context")
     ap.add_argument("--train-pairs", type=int, default=2000)
     ap.add_argument("--eval-steps", type=int, default=10)
     args = ap.parse_args()
@@ -368,7 +372,7 @@ def bench_ensemble(qx, args):
     dx0 = 10.0 / 256
     ds = [1e-3 * ((L / N) / dx0) ** 2 for N, L in zip(Ns, Ls)]
     steps = args.steps if args.steps != 50 else 500
-    ens = qx.Ensemble(Ns, Ls, precision="fp32", pref=1.0, graph_steps=16)
+    ens = qx.Ensemble(Ns, Ls, precision="fp32", pref=1.0, graph_steps=args.graph_steps)
     ens.ic_grf(list(range(len(Ns))))
     ens.update_potential()
     variants = ens.tune(ds, reps=3) if not args.no_tune else None
@@ -382,6 +386,26 @@ def bench_ensemble(qx, args):
     ms = ev.elapsed_ms(e0, e1)
     m1 = ens.mass()
     S = ens.sites
+    solo = None
+    if args.compare_solo:
+        # the same members one lattice at a time, each in its own context
+        # (its own launches; graph replays where the context uses them)
+        tot = 0.0
+        for i, (N, L) in enumerate(zip(Ns, Ls)):
+            lat = qx.Lattice(2, N, L, "fp32", pref=1.0)
+            lat.ic_grf(i)
+            lat.update_potential()
+            lat.leapfrog(ds[i], max(args.warmup, 1))
+            f0, f1 = ev.event(), ev.event()
+            ev.record(f0, lat.stream)
+            lat.leapfrog(ds[i], steps)
+            ev.record(f1, lat.stream)
+            tot += ev.elapsed_ms(f0, f1)
+            lat.close()
+        solo = {"ms_per_step": tot / steps, "value": S * steps / (tot * 1e-3),
+                "speedup_of_ensemble": tot / ms}
+        print(f"[bench] members alone: {tot / steps:.4f} ms/step ({S * steps / (tot * 1e-3):.3e} "
+              f"site-updates/s); the ensemble is {tot / ms:.2f}x that", file=sys.stderr, flush=True)
     c, r = 8, 4
     B = sum(((9 * c + 5 * r) + 4 * c * (N // 2 + 1) / N) * N * N for N in Ns)
     out = {"metric": "ensemble site-updates/s (96 uneven 2D lattices, FP32)",
@@ -389,12 +413,13 @@ def bench_ensemble(qx, args):
            "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic: per-member GRF ICs, seeds 0-95",
            "config": {"workload": "32x128^2 + 32x512^2 + 24x1024^2 + 8x2048^2, L~U[5,40], "
-                                  "dt=1e-3 (dx/dx0)^2, hipGraph 16 steps/replay",
+                                  f"dt=1e-3 (dx/dx0)^2, hipGraph {args.graph_steps} steps/replay",
                       "tile_variants": {str(k): v for k, v in variants.items()} if variants else None},
            "step_roofline": {"bound": "hbm", "algorithmic_bytes": B,
                              "achieved": B / (ms / steps * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                              "unit": "GB/s", "frac": B / (ms / steps * 1e-3) / HBM_PEAK},
-           "max_rel_mass_drift": float(np.max(np.abs(m1 - m0) / m0))}
+           "max_rel_mass_drift": float(np.max(np.abs(m1 - m0) / m0)),
+           "members_alone": solo}
     print(json.dumps(out), flush=True)
 
 
