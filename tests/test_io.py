@@ -84,7 +84,7 @@ def test_jaxions_scaling(h5, tmp_path):
     12-digit sqrt(2); the box size is attribute L.  The reference's test
     (tests/test_hdf5.py:22) expects 0.1^(3/2)/sqrt(2), 31.6 times the code's
     factor; it can pass only through its abs = 1e-5 tolerance on small m.
-    This build follows the code."""
+    The reader here follows the reference's code (src/io/io.cxx:264-271)."""
     rng = np.random.default_rng(5)
     N = 16
     m, v = rng.standard_normal(N * N), rng.standard_normal(N * N)
