@@ -487,14 +487,16 @@ def test_headline_512cube_fp32_against_fp64_100_steps(qx):
     assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-8
 
 
-@pytest.mark.parametrize("N,steps", [(128, 20), (512, 8)])
+@pytest.mark.parametrize("N,steps", [(128, 20), (512, 8), (1024, 2)])
 def test_headline_lattice_against_oracle(qx, N, steps):
     """The headline workload's lattice (3D, L = 40, dt = 2.5e-4, GRF seed 0,
     k = 0 zeroed) stepped on the GPU and by the FP64 oracle from the same
     state: at 512^3 this runs the paths no smaller lattice reaches (16-column
-    tiles, streaming instantiations, the 512-point Poisson pass).  FP64 within
-    1e-12 and FP32 within 1e-5 relative L2 (FP32 measured 1.4e-6 after 50
-    steps against FP64); the 128-bin P(k) of |psi|^2 at the end per bin."""
+    tiles, streaming instantiations, the 512-point Poisson pass), at 1024^3
+    the 1024-point ones (one-word twiddles, 16-column tiles under 8-column
+    strided blocks, 1024-point row kernels).  FP64 within 1e-12 and FP32
+    within 1e-5 relative L2 (FP32 measured 1.4e-6 after 50 steps against
+    FP64 at 512^3); the 128-bin P(k) of |psi|^2 at the end per bin."""
     L, dt = 40.0, 2.5e-4
     print(f"[case] 3D {N}^3 oracle IC and {steps} steps", flush=True)
     psi0 = o.ic_grf(3, N, L, 0)
