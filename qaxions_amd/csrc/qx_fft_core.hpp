@@ -135,7 +135,11 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
 };
 
 // ---------------------------------------------------------------------------
-// compile-time plan: stage 0 has radix 2, 4 or 8, the rest radix 8
+// compile-time plan: radix-8 stages, the last one of radix 2, 4 or 8.  The
+// small radix goes last: a stage's butterfly j writes positions
+// (j/Ns) Ns R + (j mod Ns) + r Ns, which for a first stage of radix 2 or 4
+// (Ns = 1) are 2 or 4 slots apart across lanes and hit few LDS banks; as the
+// last stage it writes nothing to LDS (its outputs stay in registers)
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
@@ -143,8 +147,8 @@ __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n
 __host__ __device__ constexpr int ipow8(int k) { return k == 0 ? 1 : 8 * ipow8(k - 1); }
 __host__ __device__ constexpr int plan_stages(int N) { return (ilog2(N) + 2) / 3; }
 __host__ __device__ constexpr int plan_r0(int N) { return (ilog2(N) % 3 == 0) ? 8 : ((ilog2(N) % 3 == 1) ? 2 : 4); }
-__host__ __device__ constexpr int plan_radix(int N, int s) { return N < 8 ? N : (s == 0 ? plan_r0(N) : 8); }
-__host__ __device__ constexpr int plan_ns(int N, int s) { return s == 0 ? 1 : (N < 8 ? 1 : plan_r0(N) * ipow8(s - 1)); }
+__host__ __device__ constexpr int plan_radix(int N, int s) { return N < 8 ? N : (s == plan_stages(N) - 1 ? plan_r0(N) : 8); }
+__host__ __device__ constexpr int plan_ns(int N, int s) { return (s == 0 || N < 8) ? 1 : ipow8(s); }
 
 // Stage-major twiddle table: stage s (sub-transform length Ns > 1, radix R,
 // table stride TWS = N/(Ns R)) keeps its twiddles w^{r k TWS}, r = 1..R-1,
@@ -152,7 +156,7 @@ __host__ __device__ constexpr int plan_ns(int N, int s) { return s == 0 ? 1 : (N
 // i/N}.  The lanes of a wavefront then read consecutive entries (k follows
 // the thread index), free of LDS bank conflicts, where the natural order
 // w^m put the reads of a stage r k TWS entries apart.  The table holds
-// N - R0 entries.
+// N - 8 entries (N >= 8).
 __host__ __device__ constexpr int twiddle_offset(int N, int s)
 {
     int off = 0;
@@ -342,6 +346,94 @@ __device__ __forceinline__ void fft_line2(C (&va)[EV], C (&vb)[EV], int t, C *li
                                           C *line_b, int tc, const C *tw)
 {
     stage_loop2<N, 0, DIR, L, TWM>(va, vb, t, line_a, line_b, tc, tw);
+}
+
+// Two lines in lockstep through ONE exchange line: each stage's butterflies
+// of both lines (each twiddle read from LDS serves both), then line a's
+// exchange and line b's one after the other through the same LDS area.  The
+// LDS footprint is one line's; the twiddle reads and their waits halve.
+template <int N, int S, int DIR, int TWM, typename C, int EV>
+__device__ __forceinline__ void fft_stage2s(C (&va)[EV], C (&vb)[EV], int t, C *line, const C *tw)
+{
+    using P = Plan<N, EV>;
+    using L = LdsContig;
+    constexpr int E = P::E, TPL = P::TPL;
+    constexpr int R = P::radix(S), Ns = P::ns(S), NB = E / R;
+    constexpr bool last = (S == P::NST - 1);
+    constexpr int TWO = twiddle_offset(N, S);
+    C oa[E], ob[E];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int j = t + m * TPL;
+        const int k = j & (Ns - 1);
+        C ua[R], ub[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            ua[r] = va[m + r * NB];
+            ub[r] = vb[m + r * NB];
+        }
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                ua[r] = twiddle_mul<TWM, DIR>(ua[r], tw, TWO + (r - 1) * Ns + k);
+                ub[r] = twiddle_mul<TWM, DIR>(ub[r], tw, TWO + (r - 1) * Ns + k);
+            }
+        }
+        Dft<R, DIR, C>::run(ua);
+        Dft<R, DIR, C>::run(ub);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            oa[m + r * NB] = ua[r];
+            ob[m + r * NB] = ub[r];
+        }
+    }
+    if (last) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            va[q] = oa[q];
+            vb[q] = ob[q];
+        }
+    } else {
+        // outputs of butterfly m, digit r, go to position base(m) + r Ns
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const int j = t + m * TPL;
+            const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[L::at(base + r * Ns, 1)] = oa[m + r * NB];
+        }
+        line_sync<N, L, EV>();
+#pragma unroll
+        for (int q = 0; q < E; ++q) va[q] = line[L::at(t + q * TPL, 1)];
+        line_sync<N, L, EV>();
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const int j = t + m * TPL;
+            const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[L::at(base + r * Ns, 1)] = ob[m + r * NB];
+        }
+        line_sync<N, L, EV>();
+#pragma unroll
+        for (int q = 0; q < E; ++q) vb[q] = line[L::at(t + q * TPL, 1)];
+        line_sync<N, L, EV>();
+    }
+}
+
+template <int N, int S, int DIR, int TWM, typename C, int EV>
+__device__ __forceinline__ void stage_loop2s(C (&va)[EV], C (&vb)[EV], int t, C *line, const C *tw)
+{
+    if constexpr (S < Plan<N, EV>::NST) {
+        fft_stage2s<N, S, DIR, TWM>(va, vb, t, line, tw);
+        stage_loop2s<N, S + 1, DIR, TWM>(va, vb, t, line, tw);
+    }
+}
+
+// contiguous lines only (one LDS line shared by the two)
+template <int N, int DIR, int TWM = TW_PLAIN, typename C, int EV>
+__device__ __forceinline__ void fft_line2s(C (&va)[EV], C (&vb)[EV], int t, C *line, const C *tw)
+{
+    stage_loop2s<N, 0, DIR, TWM>(va, vb, t, line, tw);
 }
 
 }  // namespace qx

@@ -69,10 +69,14 @@ template <int N, typename R> struct Geom {
     static constexpr int TL = 2 * NP;                 // rows per block
     static constexpr int ROW_THREADS = NP * PR::TPL;
     static constexpr int XCH1 = NP * PR::PADDED;                        // one line per pair
-    // staging of the half spectrum, [k][row] at pitch TL + 1 (TL >= 8): a
+    // staging of the half spectrum, [k][row] at pitch TL + 1 (TL >= 4): a
     // wave reading one row at 64 consecutive k then spreads over all banks
     // (at pitch TL = 16, 128 bytes, every lane hits the same bank pair)
-    static constexpr int STP = TL >= 8 ? TL + 1 : TL;
+    static constexpr int STP = TL >= 4 ? TL + 1 : TL;
+    // r2c_rows: a block's segment of each k-row of vt is TL values; where
+    // that is less than a 128 B line, the blocks that write the rest of the
+    // line run next on the same XCD (xcd_block), so its L2 merges the parts
+    static constexpr bool R2C_XCD = TL * (int)sizeof(C) < 128;
     static constexpr int STG = (N / 2 + 1) * STP;     // staging slots
     static constexpr int ROW_LDS1 = XCH1 > STG ? XCH1 : STG;    // r2c_rows
 
@@ -94,7 +98,7 @@ template <int N, typename R> struct Geom {
     static constexpr bool TWO_LINES = !WAVE_LINE && !PAIR_LINE &&
         (long long)(2 * NPC * PR::PADDED + N) * (long long)sizeof(C) + 256 <= 81920;
     static constexpr int XCH = (TWO_LINES ? 2 : 1) * NPC * PR::PADDED;  // exchange slots
-    static constexpr int STPC = TLC >= 8 ? TLC + 1 : TLC;
+    static constexpr int STPC = TLC >= 4 ? TLC + 1 : TLC;
     static constexpr int STGC = (N / 2 + 1) * STPC;
     static constexpr int ROW_LDS = XCH > STGC ? XCH : STGC;
     // occupancy of the latency-limited FP32 c2r_rows: one-wavefront lines 6
@@ -102,6 +106,10 @@ template <int N, typename R> struct Geom {
     // 5 waves with none); two-wavefront lines 5 (90 VGPRs; 6 spilled 25 and
     // took 6.00 against 5.54 ms at 1024^3)
     static constexpr int C2R_WAVES = sizeof(C) != 8 ? 1 : (WAVE_LINE ? 6 : (PAIR_LINE ? 5 : 1));
+    // one-wavefront lines: the drift transforms a pair's two rows in
+    // lockstep through the pair's one exchange line (fft_line2s), so each
+    // twiddle read serves both rows
+    static constexpr bool SHARED_LOCKSTEP = WAVE_LINE;
 };
 
 // pointwise factor in fused passes
@@ -764,7 +772,7 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
-    const long long r0 = (long long)blockIdx.x * TL;
+    const long long r0 = (G::R2C_XCD ? xcd_block() : (long long)blockIdx.x) * TL;
     const long long ra = r0 + 2 * p, rb = ra + 1;
     const bool oka = ra < rows, okb = rb < rows;
     C *mine = lds + p * P::PADDED;
@@ -1031,6 +1039,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             if (!(flags & C2R_SRC_VT)) __syncthreads();
             if constexpr (G::TWO_LINES) {
                 fft_line2<N, -1, LdsContig, TWD>(a, b, t, mine, mine_b, 1, tw);
+            } else if constexpr (G::SHARED_LOCKSTEP) {
+                fft_line2s<N, -1, TWD>(a, b, t, mine, tw);
             } else {
                 fft_line<N, -1, LdsContig, TWD>(a, t, mine, 1, tw);
                 fft_line<N, -1, LdsContig, TWD>(b, t, mine, 1, tw);
@@ -1056,6 +1066,8 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
             }
             if constexpr (G::TWO_LINES) {
                 fft_line2<N, +1, LdsContig, TWD>(a, b, t, mine, mine_b, 1, tw);
+            } else if constexpr (G::SHARED_LOCKSTEP) {
+                fft_line2s<N, +1, TWD>(a, b, t, mine, tw);
             } else {
                 fft_line<N, +1, LdsContig, TWD>(a, t, mine, 1, tw);
                 fft_line<N, +1, LdsContig, TWD>(b, t, mine, 1, tw);
