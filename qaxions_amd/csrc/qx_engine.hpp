@@ -39,7 +39,17 @@ struct Engine {
     // [k1][i0]); `units` of work (column tiles or line blocks) for the grid
     virtual long long bin_units(int dim) const = 0;
     virtual void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) = 0;
-    void *tw = nullptr;  // device twiddles in stage-major order, N entries (FP32: then their low words, see cmul2)
+    void *tw = nullptr;  // device twiddles in stage-major order, N entries; FP32 engines then hold
+                         // their N low words (see cmul2; lines up to 512 points) or DITHER_K
+                         // dithered tables of N entries (twd) and the drift counter (tick)
+    void *twd = nullptr;
+    unsigned *tick = nullptr;
+    // the next drift takes the next dithered table: once after each drift
+    // (all of its axes' passes), in stream order
+    void advance(hipStream_t s)
+    {
+        if (tick) tick_advance<unsigned><<<1, 64, 0, s>>>(tick);
+    }
     // streaming loads and stores in the passes (lattices whose arrays are far
     // larger than the memory-side cache; instantiated for N >= 512 only)
     bool stream_io = false;
@@ -93,7 +103,7 @@ template <int N, typename R> struct EngineT final : Engine {
         const long long ntiles = inner / TC;
         const unsigned blocks = (unsigned)(outer * ntiles);
         strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_STORE_TILES, CPT, TT, NT><<<blocks, (TC / CPT) * P::TPL, 0, s>>>(
-            (C *)data, inner, ntiles, m, (const C *)tw, (C *)tiles, outer);
+            (C *)data, inner, ntiles, m, (const C *)tw, (C *)tiles, outer, (const C *)twd, tick);
     }
     void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner, const MulArgs &m,
                              hipStream_t s) override
@@ -112,11 +122,12 @@ template <int N, typename R> struct EngineT final : Engine {
         const int threads = TC * P::TPL;
         C *d = (C *)data;
         const C *t = (const C *)tw;
-        if (op < 0) strided_fft<N, R, OP_FWD, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
-        else if (op > 0) strided_fft<N, R, OP_INV, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+        const C *td = (const C *)twd;
+        if (op < 0) strided_fft<N, R, OP_FWD, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0, td, tick);
+        else if (op > 0) strided_fft<N, R, OP_INV, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0, td, tick);
         else if (m.kind == MUL_DRIFT)
-            strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
-        else strided_fft<N, R, OP_POISSON, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0);
+            strided_fft<N, R, OP_DRIFT_AXIS, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0, td, tick);
+        else strided_fft<N, R, OP_POISSON, TC, XM_NONE, 1, TC, NT><<<blocks, threads, 0, s>>>(d, inner, ntiles, m, t, nullptr, 0, td, tick);
     }
     template <bool NT>
     void strided_t(void *data, long long outer, long long inner, int op, const MulArgs &m, hipStream_t s)
@@ -178,7 +189,7 @@ template <int N, typename R> struct EngineT final : Engine {
             if (stream_io) k = qx::c2r_rows<N, R, true>;
         k<<<blocks, G::ROW_THREADS_C, 0, s>>>((const C *)vt, (R *)V, (C *)psi, rows, rpm, (R)kick_dt, kick_member,
                                              flags, vmax, (const C *)tw, (const C *)tiles, ptab, ftab, ptab_stride,
-                                             inv_n);
+                                             inv_n, (const C *)twd, tick);
     }
 };
 

@@ -74,15 +74,47 @@ std::vector<int> twiddle_order(int N)
     return order;
 }
 
-// Twiddles e^{-2 pi i m/N}, computed in FP64.  In FP32 each is rounded to
-// the pair of neighbouring floats (cos and sin each rounded down or up) whose
-// squared modulus is closest to 1: nearest rounding of the two parts leaves
-// |w|^2 - 1 with a nonzero mean over the table (-4e-9 at N = 256), a bias
-// every twiddled multiply would add to the norm (mass) of psi.  FP32 tables
-// hold N more entries after those: the remainders w - wh of the rounded
-// twiddles wh, rounded, which the drift's transforms add in (two-word
-// products, cmul2).  Both are stored in stage-major order (twiddle_order).
-qx_status upload_twiddles(void *dst, int N, bool fp64)
+// FP32 lines longer than 512 points: dithered drift tables (Dither)
+bool dithered(int N, bool fp64) { return !fp64 && N > 512; }
+
+// bytes of an engine's twiddle allocation: FP64 N double2; FP32 N float2,
+// then N float2 low words (lines up to 512 points) or DITHER_K dithered
+// tables of N float2 and the drift counter (longer lines)
+size_t twiddle_bytes(int N, bool fp64)
+{
+    return dithered(N, fp64) ? 8 * (size_t)N * (DITHER_K + 1) + 256 : 16 * (size_t)N;
+}
+
+// a deterministic value in [-1/2, 1/2) for entry i, component j of the
+// N-point tables: the starting phase of that component's sigma-delta
+// sequence (splitmix64 of the three), so that the components' sequences are
+// not in step with each other
+double dither_phase(int N, int i, int j)
+{
+    uint64_t z = ((uint64_t)N << 40) ^ ((uint64_t)i << 2) ^ (uint64_t)j;
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53 - 0.5;
+}
+
+// Twiddles e^{-2 pi i m/N}, computed in FP64, in stage-major order
+// (twiddle_order), uploaded to the engine's allocation (twiddle_bytes).
+// FP64: the table.  FP32: the plain table, each twiddle rounded to the pair
+// of neighbouring floats (cos and sin each rounded down or up) whose squared
+// modulus is closest to 1 (nearest rounding of the two parts leaves
+// |w|^2 - 1 with a nonzero mean over the table, -4e-9 at N = 256, a bias
+// every twiddled multiply would add to the norm of the data).  Then, for
+// lines up to 512 points, the remainders w - wh of those rounded twiddles
+// wh, rounded, which the drift's transforms add in (two-word products,
+// cmul2); for longer lines the drift's DITHER_K tables: table s holds each
+// component as the lower or the upper of its two neighbouring floats
+// x_lo < x < x_hi, chosen by a sigma-delta sequence (acc += p with
+// p = (x - x_lo)/(x_hi - x_lo), x_hi where acc reaches 1/2, which then drops
+// by one), so that over any run of tables the count of x_hi follows p times
+// the run's length within one; then the drift counter, zeroed.
+qx_status upload_twiddles(Engine *eng, int N, bool fp64)
 {
     std::vector<double> nat(2 * (size_t)N);
     for (int m = 0; m < N; ++m) {
@@ -103,9 +135,12 @@ qx_status upload_twiddles(void *dst, int N, bool fp64)
     }
     hipError_t e;
     if (fp64) {
-        e = hipMemcpy(dst, tw.data(), 16 * (size_t)N, hipMemcpyHostToDevice);
+        e = hipMemcpy(eng->tw, tw.data(), 16 * (size_t)N, hipMemcpyHostToDevice);
+        eng->twd = nullptr;
+        eng->tick = nullptr;
     } else {
-        std::vector<float> tf(4 * (size_t)N);
+        const bool dith = dithered(N, fp64);
+        std::vector<float> tf(2 * (size_t)N * (dith ? DITHER_K + 1 : 2), 0.0f);
         for (int m = 0; m < N; ++m) {
             const double cr = tw[2 * m], ci = tw[2 * m + 1];
             float bc = (float)cr, bs = (float)ci;
@@ -120,12 +155,30 @@ qx_status upload_twiddles(void *dst, int N, bool fp64)
                 }
             tf[2 * m] = bc;
             tf[2 * m + 1] = bs;
-            // low words: the remainder of each part, for two-word products
-            // (cmul2)
-            tf[2 * (size_t)N + 2 * m] = (float)(cr - (double)bc);
-            tf[2 * (size_t)N + 2 * m + 1] = (float)(ci - (double)bs);
+            if (!dith) {
+                tf[2 * (size_t)N + 2 * m] = (float)(cr - (double)bc);
+                tf[2 * (size_t)N + 2 * m + 1] = (float)(ci - (double)bs);
+            }
         }
-        e = hipMemcpy(dst, tf.data(), 16 * (size_t)N, hipMemcpyHostToDevice);
+        for (size_t i = 0; dith && i < order.size(); ++i)
+            for (int j = 0; j < 2; ++j) {
+                const double x = tw[2 * i + j];
+                float lo = (float)x;
+                if ((double)lo > x) lo = std::nextafterf(lo, -2.f);
+                const float hi = (double)lo == x ? lo : std::nextafterf(lo, 2.f);
+                const double p = hi == lo ? 0.0 : (x - (double)lo) / ((double)hi - (double)lo);
+                double acc = dither_phase(N, (int)i, j);
+                for (int t = 0; t < DITHER_K; ++t) {
+                    acc += p;
+                    const bool up = acc >= 0.5;
+                    if (up) acc -= 1.0;
+                    tf[2 * (size_t)N * (1 + t) + 2 * i + j] = up ? hi : lo;
+                }
+            }
+        e = hipMemcpy(eng->tw, tf.data(), sizeof(float) * tf.size(), hipMemcpyHostToDevice);
+        eng->twd = dith ? (char *)eng->tw + 8 * (size_t)N : nullptr;
+        eng->tick = dith ? (unsigned *)((char *)eng->tw + 8 * (size_t)N * (DITHER_K + 1)) : nullptr;
+        if (e == hipSuccess && dith) e = hipMemset(eng->tick, 0, sizeof(unsigned));
     }
     if (e != hipSuccess) return fail(QX_EDEVICE, "twiddle upload failed");
     return QX_OK;
@@ -423,6 +476,7 @@ qx_status drift_core(qx_ctx *c, double dt)
     } else {
         c->eng->strided(c->psi, 1, N, 0, am, c->stream);
     }
+    c->eng->advance(c->stream);
     return QX_OK;
 }
 void vt_forward(qx_ctx *c, void *vt)
@@ -524,6 +578,25 @@ qx_status reduce(qx_ctx *c, const void *psi, const void *V, long long n, double 
 bool valid_ctx(const qx_ctx *c) { return c != nullptr && c->eng != nullptr; }
 
 // ---- the fused steady-state step, pass by pass ---------------------------
+// the drift counter around work on scratch copies of the state (pass
+// timing, tuning): read before, written back after, so that the steps that
+// follow take the tables they would have taken without that work
+qx_status tick_read(Engine *eng, hipStream_t s, unsigned *v)
+{
+    *v = 0;
+    if (!eng->tick) return QX_OK;
+    HIPCHK(hipMemcpyAsync(v, eng->tick, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return QX_OK;
+}
+qx_status tick_write(Engine *eng, hipStream_t s, unsigned v)
+{
+    if (!eng->tick) return QX_OK;
+    HIPCHK(hipMemcpyAsync(eng->tick, &v, sizeof(unsigned), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return QX_OK;
+}
+
 int step_pass_count(const qx_ctx *c) { return c->dim == 3 ? 7 : 4; }
 
 // pass p of one fused KDK step on psi (row layout), its column tiles psit,
@@ -553,6 +626,7 @@ void step_pass(qx_ctx *c, int p, double ds, double pref, void *psi, void *psit, 
         case 2:
             if (t) c->eng->strided(psit, N / TC, N * TC, 0, am, c->stream);
             else c->eng->strided(psi, 1, (long long)N * N, 0, am, c->stream);
+            c->eng->advance(c->stream);   // the step's drift is complete
             break;
         case 3:
             if (t) c->eng->r2c_rows(R2C_DENSITY_TILES, psit, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream);
@@ -569,7 +643,10 @@ void step_pass(qx_ctx *c, int p, double ds, double pref, void *psi, void *psit, 
             c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, nullptr,
                              am.ptab, am.ftab, 0, inv_n);
             break;
-        case 1: c->eng->strided(psi, 1, N, 0, am, c->stream); break;
+        case 1:
+            c->eng->strided(psi, 1, N, 0, am, c->stream);
+            c->eng->advance(c->stream);   // the step's drift is complete
+            break;
         case 2: c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, c->rows, pref, c->ph.rho_bar, c->stream); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
@@ -657,7 +734,7 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
     if ((s = ensure(c.get(), &c->psi, (size_t)c->S * c->csz)) != QX_OK) return cleanup(s);
     if ((s = ensure(c.get(), &c->V, (size_t)c->S * c->rsz)) != QX_OK) return cleanup(s);
     if ((s = ensure(c.get(), &c->vt, (size_t)c->H * c->csz)) != QX_OK) return cleanup(s);
-    if ((s = ensure(c.get(), &c->eng->tw, (size_t)N * 16)) != QX_OK) return cleanup(s);
+    if ((s = ensure(c.get(), &c->eng->tw, twiddle_bytes(N, prec == QX_FP64))) != QX_OK) return cleanup(s);
     if ((s = ensure(c.get(), (void **)&c->partials, sizeof(double) * 2 * MAX_PART)) != QX_OK)
         return cleanup(s);
     if ((s = ensure(c.get(), (void **)&c->vmax_bits, sizeof(unsigned long long))) != QX_OK)
@@ -667,7 +744,7 @@ qx_status qx_create(int dim, int N, double L, qx_precision prec, int device, qx_
     double *scr = nullptr;  // qx_density_stats_noalloc's scratch on this device
     if ((s = scratch_address(device, &scr)) != QX_OK) return cleanup(s);
 
-    if (upload_twiddles(c->eng->tw, N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
+    if (upload_twiddles(c->eng.get(), N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
     if (hipMemsetAsync(c->psi, 0, (size_t)c->S * c->csz, c->stream) != hipSuccess ||
         hipMemsetAsync(c->V, 0, (size_t)c->S * c->rsz, c->stream) != hipSuccess ||
         hipMemsetAsync(c->vt, 0, (size_t)c->H * c->csz, c->stream) != hipSuccess ||
@@ -746,6 +823,10 @@ qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src)
     HIPCHK(hipMemcpyAsync(dst->psi, src->psi, (size_t)src->S * src->csz, hipMemcpyDeviceToDevice, dst->stream));
     HIPCHK(hipMemcpyAsync(dst->V, src->V, (size_t)src->S * src->rsz, hipMemcpyDeviceToDevice, dst->stream));
     HIPCHK(hipMemcpyAsync(dst->vt, src->vt, (size_t)src->H * src->csz, hipMemcpyDeviceToDevice, dst->stream));
+    // the drift counter is part of the state: the copy's next drift takes
+    // the source's next table
+    if (dst->eng->tick && src->eng->tick)
+        HIPCHK(hipMemcpyAsync(dst->eng->tick, src->eng->tick, sizeof(unsigned), hipMemcpyDeviceToDevice, dst->stream));
     dst->vmax = src->vmax;
     return sync(dst);
 }
@@ -1279,6 +1360,8 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     CHECK_CTX(c);
     const int np = step_pass_count(c);
     if (reps < 1 || !ms || cap < np) return fail(QX_EINVAL, "need reps >= 1 and room for every pass");
+    unsigned tick0 = 0;
+    TRY(tick_read(c->eng.get(), c->stream, &tick0));
     TRY(ensure(c, &c->aux_c, (size_t)c->S * c->csz));
     TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
     HIPCHK(hipMemcpyAsync(c->aux_c, c->psi, (size_t)c->S * c->csz, hipMemcpyDeviceToDevice, c->stream));
@@ -1319,6 +1402,7 @@ qx_status qx_time_step_passes(qx_ctx *c, double ds, int reps, double *ms, double
     for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
     if (st != QX_OK) return st;
+    TRY(tick_write(c->eng.get(), c->stream, tick0));
     return sync(c);
 }
 
@@ -1413,6 +1497,11 @@ int64_t qx_sites(const qx_ctx *c) { return c ? c->S : 0; }
 namespace qx {
 // stream-ordered variants without the Vmax read-back (no host wait): used by
 // the corrector to chain solver steps and network passes on one stream
+// the context's drift counter (device word), which picks each drift's
+// dithered twiddle table; null where the lattice has no dithered tables
+// (FP64, and FP32 lines of at most 512 points)
+unsigned *drift_tick(qx_ctx *c) { return c->eng->tick; }
+
 qx_status leapfrog_enqueue(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)
 {
     if (nsteps == 0) return fail(QX_EINVAL, "nsteps must be >= 1");
@@ -1638,6 +1727,7 @@ void ens_class_step(qx_ens *e, EnsClass &k, hipStream_t s)
     k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_DRIFT,
                     nullptr, s, nullptr, k.d_ptab, ens_ftab(k), k.N, 1.0 / k.N);
     k.eng->strided(k.psi, k.B, k.N, 0, dm, s);
+    k.eng->advance(s);
     k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
     k.eng->contig(k.vt, k.B * k.H1, 0, pm, s);
 }
@@ -1750,7 +1840,7 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
         void **bufs[] = {&k.psi, &k.V, &k.vt, &k.eng->tw, (void **)&k.d_ptab, (void **)&k.d_pfac,
                          (void **)&k.d_kick_first, (void **)&k.d_kick_mid, (void **)&k.d_kick_last};
         const size_t sz[] = {(size_t)(k.B * k.S) * e->csz, (size_t)(k.B * k.S) * e->rsz,
-                             (size_t)(k.B * k.H) * e->csz, (size_t)k.N * 16,
+                             (size_t)(k.B * k.H) * e->csz, twiddle_bytes(k.N, prec == QX_FP64),
                              (sizeof(double2) + sizeof(float2)) * k.B * k.N, sizeof(double) * k.B, sizeof(double) * k.B,
                              sizeof(double) * k.B, sizeof(double) * k.B};
         for (int b = 0; b < 9; ++b)
@@ -1758,7 +1848,7 @@ qx_status qx_ens_create(int n, const int *Ns, const double *Ls, qx_precision pre
                 *bufs[b] = nullptr;
                 return cleanup(fail(QX_ENOMEM, "ensemble allocation failed"));
             }
-        if (upload_twiddles(k.eng->tw, k.N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
+        if (upload_twiddles(k.eng.get(), k.N, prec == QX_FP64) != QX_OK) return cleanup(QX_EDEVICE);
         if (hipMemset(k.psi, 0, sz[0]) != hipSuccess || hipMemset(k.V, 0, sz[1]) != hipSuccess ||
             hipMemset(k.vt, 0, sz[2]) != hipSuccess)
             return cleanup(fail(QX_EDEVICE, "state initialisation failed"));
@@ -1920,6 +2010,8 @@ qx_status qx_ens_tune(qx_ens *e, const double *ds, int reps, int *variants)
         EnsClass &k = e->classes[ci];
         const size_t bp = (size_t)(k.B * k.S) * e->csz, bv = (size_t)(k.B * k.H) * e->csz;
         void *sp = nullptr, *sv = nullptr;
+        unsigned tick0 = 0;
+        if (st == QX_OK) st = tick_read(k.eng.get(), e->stream, &tick0);
         if (hipMalloc(&sp, bp) != hipSuccess || hipMalloc(&sv, bv) != hipSuccess) {
             st = fail(QX_ENOMEM, "tuning copies");
         } else if (hipMemcpyAsync(sp, k.psi, bp, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
@@ -1944,6 +2036,7 @@ qx_status qx_ens_tune(qx_ens *e, const double *ds, int reps, int *variants)
                 hipMemcpyAsync(k.vt, sv, bv, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
                 hipStreamSynchronize(e->stream) != hipSuccess)
                 st = fail(QX_EDEVICE, "restoring state after tuning");
+            if (st == QX_OK) st = tick_write(k.eng.get(), e->stream, tick0);
         }
         (void)hipStreamSynchronize(e->stream);
         if (sp) (void)hipFree(sp);
@@ -1986,6 +2079,7 @@ qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_hal
         k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_DRIFT, nullptr,
                         e->stream, nullptr, k.d_ptab, ens_ftab(k), k.N, 1.0 / k.N);
         k.eng->strided(k.psi, k.B, k.N, 0, dm, e->stream);
+        k.eng->advance(e->stream);
         k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
         k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
     }

@@ -36,6 +36,7 @@
 namespace qx {
 qx_status leapfrog_enqueue(qx_ctx *c, double ds, int nsteps, int first_half, int last_half);
 qx_status update_potential_enqueue(qx_ctx *c);
+unsigned *drift_tick(qx_ctx *c);
 }  // namespace qx
 
 namespace {
@@ -484,6 +485,7 @@ struct qx_net {
     bool packed = false;
     int wchunk = 512, wblocks = 0;
     void *save_psi = nullptr, *save_V = nullptr;  // fallback copies (lazy)
+    unsigned *save_tick = nullptr;                // and of the lattice's drift counter
     double energy_tol = 0.0;  // coarse_steps: energy-drift gate per chunk (0: off)
 };
 
@@ -646,7 +648,7 @@ qx_status qx_net_destroy(qx_net *n)
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     void *b[] = {n->theta, n->grad, n->m, n->v, n->wf, n->wt, n->z[0], n->z[1], n->z[2], n->a[0], n->a[1],
                  n->a[2], n->y, n->d0, n->d1, n->partial, n->lpart, n->xbuf, n->tbuf, n->xin, n->save_psi,
-                 n->save_V};
+                 n->save_V, n->save_tick};
     for (void *p : b)
         if (p) (void)hipFree(p);
     if (n->h_lpart) (void)hipHostFree(n->h_lpart);
@@ -868,6 +870,11 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
     const size_t bp = sizeof(float) * n->P * 2, bv = sizeof(float) * n->P;
     if (!n->save_psi && hipMalloc(&n->save_psi, bp) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
     if (!n->save_V && hipMalloc(&n->save_V, bv) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
+    if (!n->save_tick && hipMalloc((void **)&n->save_tick, 256) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
+    // the drift counter (lines of more than 512 points) picks each drift's
+    // twiddle table: a redone chunk starts from the saved one, so that it
+    // equals plain fine stepping
+    unsigned *tick = qx::drift_tick(ctx);
     hipStream_t s = (hipStream_t)qx_stream(ctx);
     int fb = 0;
     double m0 = 0.0, target = 0.0, e0 = 0.0;
@@ -885,6 +892,7 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
         const int chunk = std::min(check_every, nsteps - done);
         NCHK(hipMemcpyAsync(n->save_psi, psi, bp, hipMemcpyDeviceToDevice, s));
         NCHK(hipMemcpyAsync(n->save_V, V, bv, hipMemcpyDeviceToDevice, s));
+        if (tick) NCHK(hipMemcpyAsync(n->save_tick, tick, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
         for (int i = 0; i < chunk; ++i) {
             if (qx::leapfrog_enqueue(ctx, k * dt, 1, 1, 1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
             lattice_input(n, psi, V, s);
@@ -901,6 +909,7 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
             ++fb;
             NCHK(hipMemcpyAsync(psi, n->save_psi, bp, hipMemcpyDeviceToDevice, s));
             NCHK(hipMemcpyAsync(V, n->save_V, bv, hipMemcpyDeviceToDevice, s));
+            if (tick) NCHK(hipMemcpyAsync(tick, n->save_tick, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
             if (qx_leapfrog(ctx, dt, k * chunk, 1, 1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
             if (qx_density_stats(ctx, nullptr, &m1) != QX_OK) return nfail(QX_EDEVICE, qx_last_error());
             if (egate) NTRY(energy(&e1));

@@ -66,11 +66,15 @@ def test_adam_step_lowers_loss(qx):
     assert l1 < 0.5 * l0
 
 
-def test_forced_fallback_equals_fine_steps(qx):
+@pytest.mark.parametrize("N", [64, 1024])
+def test_forced_fallback_equals_fine_steps(qx, N):
     """mass_tol < 0 forces every chunk back to fine stepping: the result is
-    exactly k*n fine steps (the failure path, forced from the host)."""
+    exactly k*n fine steps (the failure path, forced from the host).  At
+    N = 1024 the drift's dithered tables follow the lattice's drift counter,
+    which a redone chunk restores with the state."""
     from qaxions_amd.learned import init_params
-    N, L, dt, k = 64, 10.0, 1e-3, 4
+    L, dt, k = 10.0, 1e-3, 4
+    print(f"[case] forced fallback N={N}", flush=True)
     net = qx.Corrector(N, N, params=init_params())
     a = qx.Lattice(2, N, L, "fp32", pref=1.0)
     a.ic_grf(5)

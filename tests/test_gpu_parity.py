@@ -327,7 +327,8 @@ def test_mre_schedule_matches_oracle(qx, dim, N, L, steps, prec, tol):
     assert lat.vmax == pytest.approx(np.abs(Vw).max(), rel=10 * tol)
 
 
-@pytest.mark.parametrize("dim,N,prec,ns", [(2, 64, "fp64", 170), (3, 32, "fp32", 100), (3, 8, "fp32", 20)])
+@pytest.mark.parametrize("dim,N,prec,ns", [(2, 64, "fp64", 170), (3, 32, "fp32", 100), (3, 8, "fp32", 20),
+                                         (2, 1024, "fp32", 12)])
 def test_constant_schedule_is_leapfrog_bitwise(qx, dim, N, prec, ns):
     """Constant per-step values run the same kernels with the same tables as
     qx_leapfrog (graph-captured there for these sizes).  170 steps take six
@@ -567,6 +568,32 @@ def test_1024cube_fp32_against_fp64(qx):
     assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-6
 
 
+def test_dithered_long_lines_fp32_against_fp64(qx):
+    """FP32 lines of more than 512 points take dithered one-word twiddle
+    tables in the drift (one table per drift, picked by the lattice's drift
+    counter): a rounded table's error would come back every step and drift
+    the mass and phase linearly; over the dithered tables it stays bounded.
+    2D 2048^2 GRF, 400 steps, FP32 against FP64 from the same IC."""
+    N, L, dt, steps = 2048, 40.0, 2.5e-4, 400
+    out, drift = {}, {}
+    for prec in ("fp64", "fp32"):
+        print(f"[case] 2D {N}^2 {prec} {steps} steps", flush=True)
+        lat = qx.Lattice(2, N, L, prec, pref=1.0, rho_bar=1.0, k0_mode=0)
+        lat.ic_grf(4, 0.1)
+        lat.update_potential()
+        _, m0 = lat.density_stats()
+        lat.leapfrog(dt, steps // 2)
+        lat.leapfrog(dt, steps // 2)
+        _, m1 = lat.density_stats()
+        out[prec] = lat.psi
+        drift[prec] = abs(m1 - m0) / m0
+        lat.close()
+    err = rel_l2(out["fp32"], out["fp64"])
+    print(f"2D {N}^2 fp32 vs fp64 after {steps} steps: {err:.3e}; mass drift {drift}")
+    assert err <= 1e-5
+    assert drift["fp64"] < 1e-13 and drift["fp32"] < 1e-7
+
+
 # ------------------------------------------------------------------ initial conditions
 @pytest.mark.parametrize("dim,N", [(2, 64), (3, 16)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -712,8 +739,9 @@ def test_ensemble_tune_cache(qx, tmp_path):
 def test_pass_timing_and_tune_keep_state(qx):
     """Per-pass timing (inside consecutive steps on scratch copies) and the
     tile-variant tuner leave the state untouched, time every pass, and the
-    chosen variants do not change results."""
-    for dim, N in ((3, 32), (2, 64)):
+    chosen variants do not change results (at 2D 1024^2 also through the
+    drift counter that picks the dithered tables, which timing restores)."""
+    for dim, N in ((3, 32), (2, 64), (2, 1024)):
         print(f"[case] pass timing dim={dim} N={N}", flush=True)
         a = qx.Lattice(dim, N, 10.0, "fp32", pref=1.0)
         b = qx.Lattice(dim, N, 10.0, "fp32", pref=1.0)
