@@ -66,7 +66,10 @@ qx_status qx_download_psi(qx_ctx *ctx, double *psi_c128);
 qx_status qx_upload_V(qx_ctx *ctx, const double *V);
 qx_status qx_download_V(qx_ctx *ctx, double *V);
 /* device-to-device copy of psi, V and the potential spectrum between two
- * contexts of equal shape and precision (no reference counterpart) */
+ * contexts of equal shape and precision (no reference counterpart).  FP32
+ * lattices with lines of more than 512 points also copy their drift counter
+ * (which of the dithered twiddle tables each drift takes), so that the copy
+ * steps bit for bit like the source. */
 qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src);
 /* device pointers of psi / V (element type set by the precision) */
 qx_status qx_device_state(qx_ctx *ctx, void **psi, void **V);
@@ -194,8 +197,9 @@ qx_status qx_ic_blobs(qx_ctx *ctx, const double *table, int nblobs, double bkg);
 int qx_step_pass_count(const qx_ctx *ctx);
 const char *qx_step_pass_name(const qx_ctx *ctx, int pass);
 /* Times each pass inside `reps` consecutive fused steps (after one warm-up
- * step) on scratch copies of the state (the state itself is left
- * untouched), with HIP events between the passes on the context's stream.
+ * step) on scratch copies of the state (the state itself, drift counter
+ * included, is left untouched), with HIP events between the passes on the
+ * context's stream.
  * ms[pass] = average ms per launch; bytes[pass] (nullable) = minimum HBM
  * bytes the pass moves.  cap >= pass count. */
 qx_status qx_time_step_passes(qx_ctx *ctx, double ds, int reps, double *ms, double *bytes, int cap);

@@ -722,23 +722,31 @@ strided_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const type
     const int c = threadIdx.x % TC, t = threadIdx.x / TC;
     constexpr int TPO = N / TC;
     constexpr int TILES = (N / 2 + 1) * TPO;
+    // a tile's column values: the next tile's loads are issued once the
+    // current tile is transformed, so that they are in flight while it is
+    // binned (the transform's temporaries are dead by then)
+    auto load = [&](int b, C (&v)[P::E]) {
+        const C *src = data + (long long)(b / TPO) * N * N + (b % TPO) * TC + c;
+#pragma unroll
+        for (int q = 0; q < P::E; ++q) v[q] = src[(t + q * P::TPL) * N];
+    };
+    C v[P::E];
+    if (blockIdx.x < TILES) load(blockIdx.x, v);
     for (int b = blockIdx.x; b < TILES; b += gridDim.x) {
         const int k2 = b / TPO;
         const int k1 = (b % TPO) * TC + c;
-        const C *src = data + (long long)k2 * N * N + k1;
-        C v[P::E];
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = src[(t + q * P::TPL) * N];
         fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
         // positions E t .. E t + E - 1 of this column
 #pragma unroll
         for (int q = 0; q < P::E; ++q) lds[(t + q * P::TPL) * TC + c] = v[q];
         __syncthreads();
+        C x[P::E];
 #pragma unroll
-        for (int j = 0; j < P::E; ++j) v[j] = lds[(P::E * t + j) * TC + c];
+        for (int j = 0; j < P::E; ++j) x[j] = lds[(P::E * t + j) * TC + c];
         __syncthreads();
+        if (b + (int)gridDim.x < TILES) load(b + gridDim.x, v);
         const int n1 = foldi(k1, N);
-        bin_run<N, P::E>(hist, sbin, v, P::E * t, n1 * n1 + k2 * k2, (k2 == 0 || k2 == N / 2) ? 1.0 : 2.0);
+        bin_run<N, P::E>(hist, sbin, x, P::E * t, n1 * n1 + k2 * k2, (k2 == 0 || k2 == N / 2) ? 1.0 : 2.0);
     }
     bin_flush<N>(hist, ba);
 }
