@@ -878,18 +878,20 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
         }
     }
     __syncthreads();
-    // streamed also where a block's segment of a k-row is half a line (3D
-    // 1024^3: 3.33 against 3.49 ms plain)
+    // where a block's segment of a k-row is half a line, plain stores: the
+    // blocks holding the two halves run on one XCD (R2C_XCD), whose L2 then
+    // merges them (3D 1024^3: 2.72 against 2.87 ms streamed)
+    constexpr bool NTS = NT && !G::R2C_XCD;
     if (rpm % TL == 0) {
         C *const vb = vt + vt_block_base(r0, rpm, H1);
         for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
             const int k = i / TL, l = i % TL;
-            if (r0 + l < rows) st<NT>(vb + (long long)k * rpm + l, stg[k * G::STP + l]);
+            if (r0 + l < rows) st<NTS>(vb + (long long)k * rpm + l, stg[k * G::STP + l]);
         }
     } else {
         for (int i = threadIdx.x; i < H1 * TL; i += blockDim.x) {
             const int k = i / TL, l = i % TL;
-            if (r0 + l < rows) st<NT>(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
+            if (r0 + l < rows) st<NTS>(vt + vt_index(r0 + l, k, rpm, H1), stg[k * G::STP + l]);
         }
     }
 }
