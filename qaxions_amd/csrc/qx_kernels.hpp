@@ -106,10 +106,10 @@ template <int N, typename R> struct Geom {
     // 5 waves with none); two-wavefront lines 5 (90 VGPRs; 6 spilled 25 and
     // took 6.00 against 5.54 ms at 1024^3)
     static constexpr int C2R_WAVES = sizeof(C) != 8 ? 1 : (WAVE_LINE ? 6 : (PAIR_LINE ? 5 : 1));
-    // one-wavefront lines: the drift transforms a pair's two rows in
-    // lockstep through the pair's one exchange line (fft_line2s), so each
-    // twiddle read serves both rows
-    static constexpr bool SHARED_LOCKSTEP = WAVE_LINE;
+    // one exchange line per pair: the drift transforms a pair's two rows in
+    // lockstep through it (fft_line2s), so each twiddle read serves both
+    // rows (1024-point lines: 5.63 -> 5.27 ms at 1024^3)
+    static constexpr bool SHARED_LOCKSTEP = !TWO_LINES;
 };
 
 // pointwise factor in fused passes
@@ -700,7 +700,9 @@ __device__ __forceinline__ void bin_flush(const double *hist, const BinArgs &ba)
 template <int N, typename R> struct BinWaves {
     static constexpr int TC = Geom<N, R>::TC < N ? Geom<N, R>::TC : N;
     static constexpr int W = 2 * (TC * Plan<N>::TPL / 64) / 4;
-    static constexpr int value = N == 1024 ? (W > 8 ? 8 : (W < 1 ? 1 : W)) : 1;
+    // FP32 only: an FP64 block's LDS (its 16-byte exchange slots and twiddles)
+    // leaves one block per CU, which its registers already allow
+    static constexpr int value = (N == 1024 && sizeof(typename cplx<R>::T) == 8) ? (W > 8 ? 8 : (W < 1 ? 1 : W)) : 1;
 };
 
 template <int N, typename R>
@@ -722,19 +724,22 @@ strided_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const type
     const int c = threadIdx.x % TC, t = threadIdx.x / TC;
     constexpr int TPO = N / TC;
     constexpr int TILES = (N / 2 + 1) * TPO;
-    // a tile's column values: the next tile's loads are issued once the
-    // current tile is transformed, so that they are in flight while it is
-    // binned (the transform's temporaries are dead by then)
+    // a tile's column values: in FP32 the next tile's loads are issued once
+    // the current tile is transformed, so that they are in flight while it
+    // is binned (the transform's temporaries are dead by then; FP64 blocks
+    // have no registers to spare for them)
+    constexpr bool PF = std::is_same<R, float>::value;
     auto load = [&](int b, C (&v)[P::E]) {
         const C *src = data + (long long)(b / TPO) * N * N + (b % TPO) * TC + c;
 #pragma unroll
         for (int q = 0; q < P::E; ++q) v[q] = src[(t + q * P::TPL) * N];
     };
     C v[P::E];
-    if (blockIdx.x < TILES) load(blockIdx.x, v);
+    if (PF && blockIdx.x < TILES) load(blockIdx.x, v);
     for (int b = blockIdx.x; b < TILES; b += gridDim.x) {
         const int k2 = b / TPO;
         const int k1 = (b % TPO) * TC + c;
+        if (!PF) load(b, v);
         fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
         // positions E t .. E t + E - 1 of this column
 #pragma unroll
@@ -744,7 +749,7 @@ strided_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const type
 #pragma unroll
         for (int j = 0; j < P::E; ++j) x[j] = lds[(P::E * t + j) * TC + c];
         __syncthreads();
-        if (b + (int)gridDim.x < TILES) load(b + gridDim.x, v);
+        if (PF && b + (int)gridDim.x < TILES) load(b + gridDim.x, v);
         const int n1 = foldi(k1, N);
         bin_run<N, P::E>(hist, sbin, x, P::E * t, n1 * n1 + k2 * k2, (k2 == 0 || k2 == N / 2) ? 1.0 : 2.0);
     }
