@@ -154,8 +154,7 @@ __global__ void density_stats_finalize(const double *__restrict__ partials, int 
 
 // ---- power-spectrum binning (src/spectrum/spectrum.cxx:63-155) -----------
 // shell = round(|n|) (shell_of), bins of `spb` consecutive shells; the power
-// itself is binned inside the transform's last pass (strided_bin,
-// contig_bin)
+// itself is binned inside the transform's last pass (contig_bin)
 
 // mode counts per bin (the data-independent half of the binning): every
 // half-spectrum mode adds its weight w (1 for k_last in {0, N/2}, else 2)

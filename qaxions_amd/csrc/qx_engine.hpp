@@ -34,10 +34,11 @@ struct Engine {
                           unsigned long long *vmax, hipStream_t s, const void *tiles = nullptr,
                           const double2 *ptab = nullptr, const float2 *ftab = nullptr,
                           long long ptab_stride = 0, double inv_n = 0.0) = 0;
-    // P(k) binned inside the last forward pass of the spectrum's transform
-    // (3D: the i0 pass over vt [k2][i0][k1]; 2D: the i0 lines of vt
-    // [k1][i0]); `units` of work (column tiles or line blocks) for the grid
-    virtual long long bin_units(int dim) const = 0;
+    // P(k) binned inside the last forward pass of the spectrum's transform,
+    // along the contiguous lines (3D: the k1 lines of vt [k2][i0][k1]; 2D:
+    // the i0 lines of vt [k1][i0]); the grid: blocks of lines, at most what
+    // the GPU holds at once (the pass is a grid-stride loop)
+    virtual unsigned bin_grid(int dim, unsigned cap) = 0;
     virtual void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) = 0;
     void *tw = nullptr;  // device twiddles in stage-major order, N entries; FP32 engines then hold
                          // their N low words (see cmul2; lines up to 512 points) or DITHER_K
@@ -165,18 +166,27 @@ template <int N, typename R> struct EngineT final : Engine {
             if (stream_io) return r2c_rows_t<true>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
         r2c_rows_t<false>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
     }
-    long long bin_units(int dim) const override
+    long long bin_resident = 0;  // binning blocks the GPU holds at once (0: not yet queried)
+    static long long bin_lines(int dim) { return dim == 3 ? (long long)(N / 2 + 1) * N : N / 2 + 1; }
+    unsigned bin_grid(int dim, unsigned cap) override
     {
-        return dim == 3 ? (long long)(N / 2 + 1) * (N / (G::TC < N ? G::TC : N)) : (N / 2 + G::LPB) / G::LPB;
+        // blocks beyond what the CUs hold at once would run as a late tail
+        if (bin_resident == 0) {
+            int dev = 0, cus = 1, per_cu = 1;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, contig_bin<N, R, true>, G::CONTIG_THREADS, 0) !=
+                    hipSuccess)
+                cus = per_cu = 1;
+            bin_resident = (long long)std::max(cus, 1) * std::max(per_cu, 1);
+        }
+        const long long units = (bin_lines(dim) + G::LPB - 1) / G::LPB, resident = bin_resident;
+        return (unsigned)std::max(1LL, std::min({units, resident, (long long)cap}));
     }
     void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) override
     {
-        if (dim == 3) {
-            if constexpr (N <= 1024)
-                strided_bin<N, R><<<grid, (G::TC < N ? G::TC : N) * P::TPL, 0, s>>>((const C *)vt, ba, (const C *)tw);
-        } else {
-            contig_bin<N, R><<<grid, G::CONTIG_THREADS, 0, s>>>((const C *)vt, ba, (const C *)tw);
-        }
+        if (dim == 3) contig_bin<N, R, true><<<grid, G::CONTIG_THREADS, 0, s>>>((const C *)vt, bin_lines(3), ba, (const C *)tw);
+        else contig_bin<N, R, false><<<grid, G::CONTIG_THREADS, 0, s>>>((const C *)vt, bin_lines(2), ba, (const C *)tw);
     }
     void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm, double kick_dt,
                   const double *kick_member, int flags, unsigned long long *vmax, hipStream_t s,

@@ -692,75 +692,17 @@ __device__ __forceinline__ void bin_flush(const double *hist, const BinArgs &ba)
     for (int i = threadIdx.x; i < ba.nbins; i += blockDim.x) ba.partials[(long long)blockIdx.x * ba.nbins + i] = hist[i];
 }
 
-// 3D: vt [k2][i0][k1] after the i1 transform; columns along i0 (stride N),
-// a block's tiles of TC columns of one k2 plane in turn (grid-stride over
-// the H1 * N/TC tiles); the i0 transform then the binning
-// waves per SIMD that two blocks of strided_bin make (the LDS holds two):
-// the hint keeps the registers within that occupancy (N = 1024)
-template <int N, typename R> struct BinWaves {
-    static constexpr int TC = Geom<N, R>::TC < N ? Geom<N, R>::TC : N;
-    static constexpr int W = 2 * (TC * Plan<N>::TPL / 64) / 4;
-    // FP32 only: an FP64 block's LDS (its 16-byte exchange slots and twiddles)
-    // leaves one block per CU, which its registers already allow
-    static constexpr int value = (N == 1024 && sizeof(typename cplx<R>::T) == 8) ? (W > 8 ? 8 : (W < 1 ? 1 : W)) : 1;
-};
-
-template <int N, typename R>
-__global__ void __launch_bounds__(((Geom<N, R>::TC < N ? Geom<N, R>::TC : N) * Plan<N>::TPL))
-__attribute__((amdgpu_waves_per_eu(BinWaves<N, R>::value)))
-strided_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const typename cplx<R>::T *__restrict__ twg)
-{
-    using C = typename cplx<R>::T;
-    using P = Plan<N>;
-    // columns per block: the strided width, at most the plane's N columns
-    constexpr int TC = Geom<N, R>::TC < N ? Geom<N, R>::TC : N;
-    __shared__ C lds[N * TC];
-    __shared__ C tw[N];
-    __shared__ double hist[N / 2 + 1];
-    __shared__ short sbin[N / 2 + 2];
-    load_tw<N>(tw, twg);
-    bin_tables<N>(hist, sbin, ba);
-    __syncthreads();
-    const int c = threadIdx.x % TC, t = threadIdx.x / TC;
-    constexpr int TPO = N / TC;
-    constexpr int TILES = (N / 2 + 1) * TPO;
-    // a tile's column values: in FP32 the next tile's loads are issued once
-    // the current tile is transformed, so that they are in flight while it
-    // is binned (the transform's temporaries are dead by then; FP64 blocks
-    // have no registers to spare for them)
-    constexpr bool PF = std::is_same<R, float>::value;
-    auto load = [&](int b, C (&v)[P::E]) {
-        const C *src = data + (long long)(b / TPO) * N * N + (b % TPO) * TC + c;
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = src[(t + q * P::TPL) * N];
-    };
-    C v[P::E];
-    if (PF && blockIdx.x < TILES) load(blockIdx.x, v);
-    for (int b = blockIdx.x; b < TILES; b += gridDim.x) {
-        const int k2 = b / TPO;
-        const int k1 = (b % TPO) * TC + c;
-        if (!PF) load(b, v);
-        fft_line<N, -1, LdsCols>(v, t, lds + c, TC, tw);
-        // positions E t .. E t + E - 1 of this column
-#pragma unroll
-        for (int q = 0; q < P::E; ++q) lds[(t + q * P::TPL) * TC + c] = v[q];
-        __syncthreads();
-        C x[P::E];
-#pragma unroll
-        for (int j = 0; j < P::E; ++j) x[j] = lds[(P::E * t + j) * TC + c];
-        __syncthreads();
-        if (PF && b + (int)gridDim.x < TILES) load(b + gridDim.x, v);
-        const int n1 = foldi(k1, N);
-        bin_run<N, P::E>(hist, sbin, x, P::E * t, n1 * n1 + k2 * k2, (k2 == 0 || k2 == N / 2) ? 1.0 : 2.0);
-    }
-    bin_flush<N>(hist, ba);
-}
-
-// 2D: vt [k1][i0]; lines along i0, one per k1 (grid-stride over blocks of
-// LPB lines); the i0 transform then the binning
-template <int N, typename R>
+// Lines along the contiguous axis of the half spectrum, every other axis
+// already transformed: 3D vt [k2][i0][k1] (D3; line = k2 N + i0), 2D vt
+// [k1][i0] (line = k1).  Each wave transforms its own line (wave-local LDS
+// exchanges: no block barrier inside a line), then bins it; grid-stride
+// over blocks of LPB lines, with the next block's loads issued (FP32) once
+// the current lines are transformed, so that they are in flight while those
+// are binned.
+template <int N, typename R, bool D3>
 __global__ void __launch_bounds__((Geom<N, R>::CONTIG_THREADS))
-contig_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const typename cplx<R>::T *__restrict__ twg)
+contig_bin(const typename cplx<R>::T *__restrict__ data, long long nlines, BinArgs ba,
+           const typename cplx<R>::T *__restrict__ twg)
 {
     using C = typename cplx<R>::T;
     using P = typename Geom<N, R>::PR;
@@ -774,22 +716,36 @@ contig_bin(const typename cplx<R>::T *__restrict__ data, BinArgs ba, const typen
     __syncthreads();
     const int t = threadIdx.x % P::TPL, l = threadIdx.x / P::TPL;
     C *mine = lds + l * P::PADDED;
-    constexpr int LINES = N / 2 + 1;
-    for (int lb = blockIdx.x; lb * G_::LPB < LINES; lb += gridDim.x) {
-        const int line = lb * G_::LPB + l;
-        const bool ok = line < LINES;
-        C v[P::E];
+    const long long nlb = (nlines + G_::LPB - 1) / G_::LPB;
+    constexpr bool PF = std::is_same<R, float>::value;
+    auto load = [&](long long lb, C (&v)[P::E]) {
+        const long long line = lb * G_::LPB + l;
+        const bool ok = line < nlines;
+        const C *src = data + line * N + t;
 #pragma unroll
-        for (int q = 0; q < P::E; ++q) v[q] = ok ? data[(long long)line * N + t + q * P::TPL] : C{0, 0};
+        for (int q = 0; q < P::E; ++q) v[q] = ok ? src[q * P::TPL] : C{0, 0};
+    };
+    C v[P::E];
+    if (PF && blockIdx.x < nlb) load(blockIdx.x, v);
+    for (long long lb = blockIdx.x; lb < nlb; lb += gridDim.x) {
+        const long long line = lb * G_::LPB + l;
+        if (!PF) load(lb, v);
         fft_line<N, -1, LdsContig>(v, t, mine, 1, tw);
         // positions E t .. E t + E - 1 of this line
 #pragma unroll
         for (int q = 0; q < P::E; ++q) mine[LdsContig::at(t + q * P::TPL, 1)] = v[q];
         line_sync<N, LdsContig, P::E>();
+        C x[P::E];
 #pragma unroll
-        for (int j = 0; j < P::E; ++j) v[j] = mine[LdsContig::at(P::E * t + j, 1)];
+        for (int j = 0; j < P::E; ++j) x[j] = mine[LdsContig::at(P::E * t + j, 1)];
         line_sync<N, LdsContig, P::E>();
-        if (ok) bin_run<N, P::E>(hist, sbin, v, P::E * t, line * line, (line == 0 || line == N / 2) ? 1.0 : 2.0);
+        if (PF && lb + gridDim.x < nlb) load(lb + gridDim.x, v);
+        if (line < nlines) {
+            // the half axis' index kh (weight 1 at 0 and N/2, else 2) and the
+            // other full axis' frequency (3D)
+            const int kh = D3 ? (int)(line / N) : (int)line, no = D3 ? foldi((int)(line % N), N) : 0;
+            bin_run<N, P::E>(hist, sbin, x, P::E * t, kh * kh + no * no, (kh == 0 || kh == N / 2) ? 1.0 : 2.0);
+        }
     }
     bin_flush<N>(hist, ba);
 }

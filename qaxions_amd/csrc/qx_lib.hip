@@ -1085,19 +1085,19 @@ qx_status qx_spectrum(qx_ctx *c, int source, int nbins, int spb, double *Pk, int
     if (!Pk || !count || !kmodes) return fail(QX_EINVAL, "null output");
     const int nsh = c->N / 2 + 1;
     TRY(ensure(c, &c->aux_vt, (size_t)c->H * c->csz));
-    const unsigned grid = (unsigned)std::min<long long>(SPEC_BLOCKS, c->eng->bin_units(c->dim));
+    const unsigned grid = c->eng->bin_grid(c->dim, SPEC_BLOCKS);
     TRY(ensure(c, (void **)&c->spec_part, sizeof(double) * grid * nsh));
     TRY(ensure(c, (void **)&c->spec_P, sizeof(double) * nsh));
     TRY(ensure(c, (void **)&c->spec_cnt, sizeof(unsigned long long) * nsh));
     // the field's half spectrum: r2c along the last axis (of V, or of |psi|^2
-    // formed on the fly), then every other axis but the first; the first
-    // axis' transform bins the modes and writes nothing
+    // formed on the fly) into [k_last][rows], then (3D) the i0 axis' strided
+    // pass; the contiguous axis' transform bins the modes and writes nothing
     if (source == 0)
         c->eng->r2c_rows(R2C_REAL, nullptr, c->V, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     else
         c->eng->r2c_rows(R2C_DENSITY, c->psi, nullptr, c->aux_vt, c->rows, c->rows, 1.0, 0.0, c->stream);
     MulArgs none{};
-    if (c->dim == 3) c->eng->contig(c->aux_vt, c->H1 * c->N, -1, none, c->stream);
+    if (c->dim == 3) c->eng->strided(c->aux_vt, c->H1, c->N, -1, none, c->stream);
     const BinArgs ba{nbins, spb, c->spec_part};
     c->eng->spectrum_bin(c->dim, c->aux_vt, ba, grid, c->stream);
     bin_reduce<<<nbins, 256, 0, c->stream>>>(c->spec_part, (int)grid, nbins, c->spec_P);
