@@ -698,6 +698,44 @@ def test_ensemble_config_size_classes(qx, prec):
     assert err < (1e-12 if prec == "fp64" else 5e-6)
 
 
+def test_ensemble_96_members(qx):
+    """The batched ensemble at its full make-up: 32 x 128^2, 32 x 512^2,
+    24 x 1024^2 and 8 x 2048^2, seeds 0-95, L ~ U[5, 40], dt scaled to dx^2,
+    tuned, 16 graph-captured steps (two replays of 8).  Every member keeps its
+    mass (FP32), the first and last member of each size class equal their solo
+    runs bit for bit, and the first 128^2 member matches the FP64 oracle."""
+    from qaxions_amd.rng import uniform
+    Ns = [128] * 32 + [512] * 32 + [1024] * 24 + [2048] * 8
+    Ls = [5.0 + 35.0 * uniform(2024, 99, i) for i in range(len(Ns))]
+    dx0 = 10.0 / 256
+    ds = [1e-3 * ((L / N) / dx0) ** 2 for N, L in zip(Ns, Ls)]
+    steps = 16
+    print("[case] ensemble 96 members fp32", flush=True)
+    ens = qx.Ensemble(Ns, Ls, precision="fp32", pref=1.0, graph_steps=8)
+    ens.ic_grf(list(range(len(Ns))))
+    ens.update_potential()
+    ens.tune(ds, reps=2)
+    m0 = ens.mass()
+    ens.leapfrog(ds, steps)
+    drift = np.abs(ens.mass() / m0 - 1.0)
+    print(f"96 members after {steps} steps: worst relative mass drift {drift.max():.2e}")
+    assert drift.max() < 1e-5
+    for i in (0, 31, 32, 63, 64, 87, 88, 95):
+        solo = qx.Lattice(2, Ns[i], Ls[i], precision="fp32", pref=1.0)
+        solo.ic_grf(i)
+        solo.update_potential()
+        solo.leapfrog(ds[i], steps)
+        assert np.array_equal(ens.psi(i), solo.psi), (i, Ns[i])
+        solo.close()
+    psi0 = o.ic_grf(2, Ns[0], Ls[0], 0)
+    V0, _, _ = o.update_potential(psi0, Ls[0], 1.0)
+    want, _ = o.leapfrog(psi0, V0, Ls[0], ds[0], steps, 1.0)
+    err = rel_l2(ens.psi(0), want)
+    print(f"member 0 (128^2) against the oracle: {err:.2e}")
+    assert err < 5e-6
+    ens.close()
+
+
 def test_ensemble_tune_cache(qx, tmp_path):
     """The ensemble tuner's cache: the first tune stores {N: variant} under
     the class make-up, a second ensemble of that make-up reinstalls the
