@@ -41,7 +41,7 @@ struct Engine {
     virtual unsigned bin_grid(int dim, unsigned cap) = 0;
     virtual void spectrum_bin(int dim, const void *vt, const BinArgs &ba, unsigned grid, hipStream_t s) = 0;
     void *tw = nullptr;  // device twiddles in stage-major order, N entries; FP32 engines then hold
-                         // their N low words (see cmul2; lines up to 512 points) or DITHER_K
+                         // their N low words (see cmul2; lines under 512 points) or DITHER_K
                          // dithered tables of N entries (twd) and the drift counter (tick)
     void *twd = nullptr;
     unsigned *tick = nullptr;

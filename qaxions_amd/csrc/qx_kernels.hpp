@@ -389,7 +389,7 @@ __device__ __forceinline__ void load_tw2(float2 *tw, const float2 *twg, int n)
     }
 }
 
-// FP32 drift transforms of lines longer than 512 points (one-word
+// FP32 drift transforms of lines of 512 points and more (one-word
 // twiddles, see TwoWord) take their twiddles from one of DITHER_K tables,
 // table (tick mod DITHER_K) for the drift applied when the context's drift
 // counter reads `tick`.  Each twiddle component runs through a sigma-delta
@@ -486,20 +486,22 @@ enum { XM_NONE = 0, XM_STORE_TILES = 1 };
 // 16-column tile's 128-byte row segments with 512 threads per block).
 // NT: streaming accesses where a block row is a whole 128 B line (narrower
 // rows share lines between blocks, which then merge in L2)
-// Two-word twiddles (cmul2) in the FP32 drift transforms of lines up to 512
-// points.  Longer lines keep one word: their three twiddled stages and
-// their LDS and registers make the second word cost 11% of the 1024^3 step
-// (23.6 -> 26.3 ms), where psi's FP32 error over the 50-step workload stays
-// within 1e-5 with one word.
+// Two-word twiddles (cmul2) in the FP32 drift transforms of lines shorter
+// than 512 points.  Lines of 512 points and more take dithered one-word
+// tables instead (Dither): the second word's products cost 2.5% of the 512^3
+// step (2.30 against 2.36 ms, same box; the drift passes 442 against 476 us)
+// and 11% of the 1024^3 step (23.6 -> 26.3 ms), while the dithered tables
+// keep psi's error against FP64 where the two words put it (512^3, 50
+// steps: 1.3766e-6 against 1.3764e-6).  The mass drift is larger (3.8e-9
+// against 1.7e-10 over those 50 steps) but bounded: each twiddle's error
+// sums to about one unit in the last place over the tables.
 template <int N, typename R> struct TwoWord {
-    static constexpr bool value = std::is_same<R, float>::value && N <= 512;
+    static constexpr bool value = std::is_same<R, float>::value && N < 512;
 };
 // FP32 drift transforms of the longer lines: dithered one-word tables
-// (drift_table).  Measured at 512^3 against the two words: 1.5% faster
-// steps, but 20 times the mass drift (6e-9 against 3e-10 over 55 steps),
-// so lines up to 512 points keep the two words
+// (drift_table)
 template <int N, typename R> struct Dither {
-    static constexpr bool value = std::is_same<R, float>::value && N > 512;
+    static constexpr bool value = std::is_same<R, float>::value && N >= 512;
 };
 
 // Waves per SIMD that a strided FP32 drift block's LDS allows: its two-word

@@ -74,11 +74,11 @@ std::vector<int> twiddle_order(int N)
     return order;
 }
 
-// FP32 lines longer than 512 points: dithered drift tables (Dither)
-bool dithered(int N, bool fp64) { return !fp64 && N > 512; }
+// FP32 lines of 512 points and more: dithered drift tables (Dither)
+bool dithered(int N, bool fp64) { return !fp64 && N >= 512; }
 
 // bytes of an engine's twiddle allocation: FP64 N double2; FP32 N float2,
-// then N float2 low words (lines up to 512 points) or DITHER_K dithered
+// then N float2 low words (lines under 512 points) or DITHER_K dithered
 // tables of N float2 and the drift counter (longer lines)
 size_t twiddle_bytes(int N, bool fp64)
 {
@@ -106,7 +106,7 @@ double dither_phase(int N, int i, int j)
 // modulus is closest to 1 (nearest rounding of the two parts leaves
 // |w|^2 - 1 with a nonzero mean over the table, -4e-9 at N = 256, a bias
 // every twiddled multiply would add to the norm of the data).  Then, for
-// lines up to 512 points, the remainders w - wh of those rounded twiddles
+// lines under 512 points, the remainders w - wh of those rounded twiddles
 // wh, rounded, which the drift's transforms add in (two-word products,
 // cmul2); for longer lines the drift's DITHER_K tables: table s holds each
 // component as the lower or the upper of its two neighbouring floats
@@ -1499,7 +1499,7 @@ namespace qx {
 // the corrector to chain solver steps and network passes on one stream
 // the context's drift counter (device word), which picks each drift's
 // dithered twiddle table; null where the lattice has no dithered tables
-// (FP64, and FP32 lines of at most 512 points)
+// (FP64, and FP32 lines under 512 points)
 unsigned *drift_tick(qx_ctx *c) { return c->eng->tick; }
 
 qx_status leapfrog_enqueue(qx_ctx *c, double ds, int nsteps, int first_half, int last_half)

@@ -871,7 +871,7 @@ qx_status qx_net_coarse_steps(qx_net *n, qx_ctx *ctx, double dt, int k, int nste
     if (!n->save_psi && hipMalloc(&n->save_psi, bp) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
     if (!n->save_V && hipMalloc(&n->save_V, bv) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
     if (!n->save_tick && hipMalloc((void **)&n->save_tick, 256) != hipSuccess) return nfail(QX_ENOMEM, "save buffer");
-    // the drift counter (lines of more than 512 points) picks each drift's
+    // the drift counter (lines of 512 points and more) picks each drift's
     // twiddle table: a redone chunk starts from the saved one, so that it
     // equals plain fine stepping
     unsigned *tick = qx::drift_tick(ctx);
