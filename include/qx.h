@@ -67,7 +67,7 @@ qx_status qx_upload_V(qx_ctx *ctx, const double *V);
 qx_status qx_download_V(qx_ctx *ctx, double *V);
 /* device-to-device copy of psi, V and the potential spectrum between two
  * contexts of equal shape and precision (no reference counterpart).  FP32
- * lattices with lines of more than 512 points also copy their drift counter
+ * lattices with lines of 512 points and more also copy their drift counter
  * (which of the dithered twiddle tables each drift takes), so that the copy
  * steps bit for bit like the source. */
 qx_status qx_copy_state(qx_ctx *dst, const qx_ctx *src);

@@ -569,7 +569,7 @@ def test_1024cube_fp32_against_fp64(qx):
 
 
 def test_dithered_long_lines_fp32_against_fp64(qx):
-    """FP32 lines of more than 512 points take dithered one-word twiddle
+    """FP32 lines of 512 points and more take dithered one-word twiddle
     tables in the drift (one table per drift, picked by the lattice's drift
     counter): a rounded table's error would come back every step and drift
     the mass and phase linearly; over the dithered tables it stays bounded.
