@@ -140,7 +140,7 @@ def test_energy(qx, dim, N, L, prec):
 @pytest.mark.parametrize("dim,N,nbins,spb", [(2, 16, 9, 1), (2, 256, 64, 2), (3, 16, 9, 1),
                                              (3, 32, 8, 2), (2, 10, 6, 1), (2, 4, 3, 1), (3, 4, 3, 1),
                                              (3, 64, 33, 1), (3, 128, 21, 3), (2, 1024, 100, 5),
-                                             (2, 4096, 2049, 1)])
+                                             (2, 4096, 2049, 1), (3, 8, 5, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_spectrum(qx, dim, N, nbins, spb, prec):
     if N & (N - 1):
