@@ -81,18 +81,23 @@ template <int N, typename R> struct Geom {
     static constexpr int ROW_LDS1 = XCH1 > STG ? XCH1 : STG;    // r2c_rows
 
     // c2r_rows.  Where a line is one wavefront (TPL = 64) the kernel is
-    // latency-limited: it takes blocks of 4 waves with one exchange line per
-    // pair (27 KB of LDS with the two-word twiddles), so that 6 blocks fit a
-    // CU (C2R_WAVES).  Lines of two wavefronts (N = 1024) also take 4-wave
-    // blocks, so that each exchange's block barrier waits for 4 waves instead
-    // of 8, with one exchange line per pair (27 KB), so that 5 blocks fit
-    // (two lines in lockstep, 45 KB, left 3).  Otherwise it keeps the
-    // r2c_rows blocks and transforms a pair's two psi rows in lockstep, each
-    // with its own exchange line, where two lines per pair fit in LDS beside
-    // the twiddles (else one after the other through one line).
+    // latency-limited and its waves sync line-locally: FP32 takes blocks of 8
+    // waves (16 rows, whole 128 B lines of vt) with one exchange line per pair
+    // (41 KB of LDS), so that 3 blocks, 6 waves per SIMD, fit a CU
+    // (C2R_WAVES); 4-wave blocks of 8 rows, 6 to a CU, reading half lines of
+    // vt, took 1% longer (512^3: 567-568 against 559-561 us).  FP64 keeps the
+    // 4-wave blocks, as its 8-wave ones would take 82 KB.  Lines of two
+    // wavefronts (N = 1024) also take 4-wave blocks, so that each exchange's
+    // block barrier waits for 4 waves instead of 8, with one exchange line
+    // per pair (27 KB), so that 5 blocks fit (two lines in lockstep, 45 KB,
+    // left 3).  Otherwise it keeps the r2c_rows blocks and transforms a
+    // pair's two psi rows in lockstep, each with its own exchange line, where
+    // two lines per pair fit in LDS beside the twiddles (else one after the
+    // other through one line).
     static constexpr bool WAVE_LINE = PR::TPL == 64;
     static constexpr bool PAIR_LINE = PR::TPL == 128;
-    static constexpr int NPC = (WAVE_LINE && NP > 4) ? 4 : ((PAIR_LINE && NP > 2) ? 2 : NP);
+    static constexpr int WAVE_PAIRS = sizeof(C) == 8 ? 8 : 4;
+    static constexpr int NPC = (WAVE_LINE && NP > WAVE_PAIRS) ? WAVE_PAIRS : ((PAIR_LINE && NP > 2) ? 2 : NP);
     static constexpr int TLC = 2 * NPC;               // rows per block
     static constexpr int ROW_THREADS_C = NPC * PR::TPL;
     static constexpr bool TWO_LINES = !WAVE_LINE && !PAIR_LINE &&
@@ -902,8 +907,9 @@ c2r_rows(const typename cplx<R>::T *__restrict__ vt, R *__restrict__ V,
     else if constexpr (Dither<N, R>::value) load_tw<N>(tw, (flags & C2R_DRIFT) ? drift_table(twd, tick, N) : twg);
     else load_tw<N>(tw, twg);
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
-    // with 8-row blocks a block reads half of each 128 B line of vt: the
-    // block holding the other half runs next on the same XCD
+    // where a block's rows are less than a 128 B line of vt (FP32 blocks of
+    // fewer than 16 rows), the blocks holding the rest of the line run next
+    // on the same XCD
     const long long r0 = xcd_block() * TL;
     const long long ra = r0 + 2 * p, rb = ra + 1;
     const bool oka = ra < rows, okb = rb < rows;
