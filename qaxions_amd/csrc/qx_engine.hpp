@@ -170,15 +170,17 @@ template <int N, typename R> struct EngineT final : Engine {
     static long long bin_lines(int dim) { return dim == 3 ? (long long)(N / 2 + 1) * N : N / 2 + 1; }
     unsigned bin_grid(int dim, unsigned cap) override
     {
-        // blocks beyond what the CUs hold at once would run as a late tail
+        // blocks beyond what the CUs hold at once would run as a late tail;
+        // should the occupancy query fail, the cap stands in for it
         if (bin_resident == 0) {
-            int dev = 0, cus = 1, per_cu = 1;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, contig_bin<N, R, true>, G::CONTIG_THREADS, 0) !=
-                    hipSuccess)
-                cus = per_cu = 1;
-            bin_resident = (long long)std::max(cus, 1) * std::max(per_cu, 1);
+            int dev = 0, cus = 0, per_cu = 0;
+            const bool ok =
+                hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, contig_bin<N, R, true>, G::CONTIG_THREADS, 0) ==
+                    hipSuccess &&
+                cus > 0 && per_cu > 0;
+            bin_resident = ok ? (long long)cus * per_cu : (long long)cap;
         }
         const long long units = (bin_lines(dim) + G::LPB - 1) / G::LPB, resident = bin_resident;
         return (unsigned)std::max(1LL, std::min({units, resident, (long long)cap}));
