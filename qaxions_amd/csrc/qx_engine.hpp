@@ -180,6 +180,7 @@ template <int N, typename R> struct EngineT final : Engine {
                 hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, contig_bin<N, R, true>, G::CONTIG_THREADS, 0) ==
                     hipSuccess &&
                 cus > 0 && per_cu > 0;
+            if (!ok) (void)hipGetLastError();  // not left for a later launch check to report
             bin_resident = ok ? (long long)cus * per_cu : (long long)cap;
         }
         const long long units = (bin_lines(dim) + G::LPB - 1) / G::LPB, resident = bin_resident;
