@@ -261,7 +261,10 @@ qx_status qx_net_set_params(qx_net *net, const float *theta);  /* resets Adam st
 qx_status qx_net_set_output_scale(qx_net *net, float s);
 float qx_net_output_scale(const qx_net *net);
 /* a: the V channel's scale where the input is built from a lattice
- * (correct, coarse_steps, train_step_lattices); default 1 */
+ * (correct, coarse_steps, train_step_lattices); default 1.  a = 0 feeds the
+ * channel zeros: the network is then a function of psi alone, the 2-channel
+ * network of the 2 -> 32 -> 32 -> 32 -> 2 layout (the V channel's weights
+ * see no input and get no gradient) */
 qx_status qx_net_set_input_scale(qx_net *net, float a);
 float qx_net_input_scale(const qx_net *net);
 qx_status qx_net_get_params(qx_net *net, float *theta);

@@ -782,7 +782,7 @@ float qx_net_output_scale(const qx_net *n) { return n ? n->out_scale : 0.0f; }
 
 qx_status qx_net_set_input_scale(qx_net *n, float s)
 {
-    if (!n || !(s > 0.0f)) return nfail(QX_EINVAL, "input scale must be positive");
+    if (!n || !(s >= 0.0f)) return nfail(QX_EINVAL, "input scale must be >= 0");
     n->in_scale = s;
     return QX_OK;
 }

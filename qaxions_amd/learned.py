@@ -30,10 +30,12 @@ def init_params(seed=0, last_scale=1e-3):
 
 
 def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, log_every=0,
-          pref=1.0):
+          pref=1.0, potential=True):
     """Train a corrector on len(seeds) * steps_per_seed pairs.  Returns
     (net, losses, relative) where relative[i] = loss / mean((coarse - fine)^2)
-    of pair i: the fraction of the coarse step's error the network leaves."""
+    of pair i: the fraction of the coarse step's error the network leaves.
+    potential=False feeds the V channel zeros (input scale 0): the network
+    then sees psi alone, as a 2-channel network would."""
     if net is None:
         net = Corrector(N, N, params=init_params())
     traj = Lattice(2, N, L, "fp32", pref=pref)
@@ -57,7 +59,7 @@ def train(N, L, dt, k, seeds, steps_per_seed, lr=1e-3, lr_final=None, net=None, 
                 d = fine.psi - coarse.psi
                 net.output_scale = float(np.sqrt(np.mean(np.abs(d) ** 2) / 2.0))
                 v = coarse.V
-                net.input_scale = float(1.0 / max(np.std(v), 1e-30))
+                net.input_scale = float(1.0 / max(np.std(v), 1e-30)) if potential else 0.0
                 scale_set = True
             lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(total - 1, 1))
             base = net.pair_mse(coarse, fine)

@@ -198,3 +198,26 @@ def test_energy_gate_falls_back(qx):
     b.leapfrog(dt, k * 3)
     b.leapfrog(dt, k * 3)
     assert np.array_equal(a.psi, b.psi)
+
+
+def test_zero_input_scale_is_a_psi_only_network(qx):
+    """Input scale 0 feeds the V channel zeros: the corrected state no longer
+    depends on V (the config's 2-channel network), while with a positive
+    scale it does."""
+    from qaxions_amd.learned import init_params
+    N, L = 64, 10.0
+    out = {}
+    for a in (0.0, 1.0):
+        for vs in (1.0, 3.0):
+            net = qx.Corrector(N, N, params=init_params(last_scale=0.5))
+            net.output_scale = 0.05
+            net.input_scale = a
+            lat = qx.Lattice(2, N, L, "fp32", pref=1.0)
+            lat.ic_grf(7)
+            lat.update_potential()
+            lat.V = lat.V * vs
+            net.correct(lat)
+            out[(a, vs)] = lat.psi
+            lat.close()
+    assert np.array_equal(out[(0.0, 1.0)], out[(0.0, 3.0)])
+    assert rel(out[(1.0, 1.0)], out[(1.0, 3.0)]) > 1e-4
