@@ -24,9 +24,12 @@ struct Engine {
     virtual void strided_tiles_drift(void *data, void *tiles, long long outer, long long inner,
                                      const MulArgs &m, hipStream_t s) = 0;
     virtual int tile_cols() const = 0;
-    // rows of `rpm` rows per member (rpm = rows for a single lattice)
+    // rows of `rpm` rows per member (rpm = rows for a single lattice);
+    // advance_tick: a step's density pass, which also advances the drift
+    // counter (where the engine has one) in place of advance()
     virtual void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows,
-                          long long rpm, double pref, double rho_bar, hipStream_t s) = 0;
+                          long long rpm, double pref, double rho_bar, hipStream_t s,
+                          bool advance_tick = false) = 0;
     // C2R_SRC_TILES: psi read from `tiles`; C2R_DRIFT: the last axis' drift
     // with the per-axis table ptab (members ptab_stride apart) and 1/N
     virtual void c2r_rows(const void *vt, void *V, void *psi, long long rows, long long rpm,
@@ -145,7 +148,7 @@ template <int N, typename R> struct EngineT final : Engine {
     }
     template <bool NT>
     void r2c_rows_t(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
-                    double pref, double rho_bar, hipStream_t s)
+                    double pref, double rho_bar, hipStream_t s, unsigned *tk)
     {
         const unsigned blocks = (unsigned)((rows + G::TL - 1) / G::TL);
         C *p = (C *)psi;
@@ -153,18 +156,19 @@ template <int N, typename R> struct EngineT final : Engine {
         C *o = (C *)vt;
         const C *t = (const C *)tw;
         if (mode == R2C_REAL)
-            qx::r2c_rows<N, R, R2C_REAL, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_REAL, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, tk);
         else if (mode == R2C_DENSITY)
-            qx::r2c_rows<N, R, R2C_DENSITY, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_DENSITY, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, tk);
         else
-            qx::r2c_rows<N, R, R2C_DENSITY_TILES, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t);
+            qx::r2c_rows<N, R, R2C_DENSITY_TILES, NT><<<blocks, G::ROW_THREADS, 0, s>>>(p, vi, o, rows, rpm, (R)pref, (R)rho_bar, t, tk);
     }
     void r2c_rows(int mode, void *psi, const void *vin, void *vt, long long rows, long long rpm,
-                  double pref, double rho_bar, hipStream_t s) override
+                  double pref, double rho_bar, hipStream_t s, bool advance_tick) override
     {
+        unsigned *tk = advance_tick ? tick : nullptr;
         if constexpr (NT_OK)
-            if (stream_io) return r2c_rows_t<true>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
-        r2c_rows_t<false>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s);
+            if (stream_io) return r2c_rows_t<true>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s, tk);
+        r2c_rows_t<false>(mode, psi, vin, vt, rows, rpm, pref, rho_bar, s, tk);
     }
     long long bin_resident = 0;  // binning blocks the GPU holds at once (0: not yet queried)
     static long long bin_lines(int dim) { return dim == 3 ? (long long)(N / 2 + 1) * N : N / 2 + 1; }

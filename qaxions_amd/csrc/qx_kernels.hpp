@@ -771,7 +771,7 @@ template <int N, typename R, int MODE, bool NT>
 __global__ void __launch_bounds__((Geom<N, R>::ROW_THREADS))
 r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
          typename cplx<R>::T *__restrict__ vt, long long rows, long long rpm, R pref, R rho_bar,
-         const typename cplx<R>::T *__restrict__ twg)
+         const typename cplx<R>::T *__restrict__ twg, unsigned *__restrict__ tick)
 {
     using C = typename cplx<R>::T;
     using P = typename Geom<N, R>::PR;
@@ -780,6 +780,9 @@ r2c_rows(typename cplx<R>::T *__restrict__ psi, const R *__restrict__ vin,
     __shared__ C lds[G::ROW_LDS1];
     __shared__ C tw[N];
     load_tw<N>(tw, twg);
+    // the density pass of a step follows the step's drift: it advances the
+    // drift counter for the next drift (no pass between them reads it)
+    if (tick && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1u;
     const int t = threadIdx.x % P::TPL, p = threadIdx.x / P::TPL;
     const long long r0 = (G::R2C_XCD ? xcd_block() : (long long)blockIdx.x) * TL;
     const long long ra = r0 + 2 * p, rb = ra + 1;

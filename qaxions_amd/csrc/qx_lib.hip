@@ -626,11 +626,10 @@ void step_pass(qx_ctx *c, int p, double ds, double pref, void *psi, void *psit, 
         case 2:
             if (t) c->eng->strided(psit, N / TC, N * TC, 0, am, c->stream);
             else c->eng->strided(psi, 1, (long long)N * N, 0, am, c->stream);
-            c->eng->advance(c->stream);   // the step's drift is complete
             break;
-        case 3:
-            if (t) c->eng->r2c_rows(R2C_DENSITY_TILES, psit, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream);
-            else c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream);
+        case 3:   // after the step's drift: also advances the drift counter
+            if (t) c->eng->r2c_rows(R2C_DENSITY_TILES, psit, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream, true);
+            else c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, N, pref, c->ph.rho_bar, c->stream, true);
             break;
         case 4: c->eng->contig(vt, c->H1 * N, -1, none, c->stream, vtk, c->H1, N); break;
         case 5: c->eng->strided(vtk, c->H1, N, 0, pm, c->stream); break;
@@ -643,11 +642,10 @@ void step_pass(qx_ctx *c, int p, double ds, double pref, void *psi, void *psit, 
             c->eng->c2r_rows(vt, c->V, psi, c->rows, c->rows, ds, nullptr, kick, nullptr, c->stream, nullptr,
                              am.ptab, am.ftab, 0, inv_n);
             break;
-        case 1:
-            c->eng->strided(psi, 1, N, 0, am, c->stream);
-            c->eng->advance(c->stream);   // the step's drift is complete
+        case 1: c->eng->strided(psi, 1, N, 0, am, c->stream); break;
+        case 2:   // after the step's drift: also advances the drift counter
+            c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, c->rows, pref, c->ph.rho_bar, c->stream, true);
             break;
-        case 2: c->eng->r2c_rows(R2C_DENSITY, psi, nullptr, vt, c->rows, c->rows, pref, c->ph.rho_bar, c->stream); break;
         case 3: c->eng->contig(vt, c->H1, 0, pm, c->stream); break;
         }
     }
@@ -1727,8 +1725,7 @@ void ens_class_step(qx_ens *e, EnsClass &k, hipStream_t s)
     k.eng->c2r_rows(k.vt, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_mid, C2R_SRC_VT | C2R_KICK | C2R_DRIFT,
                     nullptr, s, nullptr, k.d_ptab, ens_ftab(k), k.N, 1.0 / k.N);
     k.eng->strided(k.psi, k.B, k.N, 0, dm, s);
-    k.eng->advance(s);
-    k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s);
+    k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, s, true);
     k.eng->contig(k.vt, k.B * k.H1, 0, pm, s);
 }
 
@@ -2079,8 +2076,8 @@ qx_status qx_ens_leapfrog(qx_ens *e, const double *ds, int nsteps, int first_hal
         k.eng->c2r_rows(nullptr, k.V, k.psi, k.rows, k.N, 0.0, k.d_kick_first, C2R_KICK | C2R_DRIFT, nullptr,
                         e->stream, nullptr, k.d_ptab, ens_ftab(k), k.N, 1.0 / k.N);
         k.eng->strided(k.psi, k.B, k.N, 0, dm, e->stream);
-        k.eng->advance(e->stream);
-        k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream);
+        k.eng->r2c_rows(R2C_DENSITY, k.psi, nullptr, k.vt, k.rows, k.N, e->ph.pref, e->ph.rho_bar, e->stream,
+                        true);
         k.eng->contig(k.vt, k.B * k.H1, 0, pm, e->stream);
     }
     TRY(ens_steady_steps(e, nsteps - 1));
